@@ -1,0 +1,182 @@
+/*
+ * sd2d.h - C ABI of the MI355X (gfx950) SD2D-MHD residual and time-step path.
+ *
+ * The reference solver keeps the whole discretisation in the Fortran module
+ * `setup2d` and calls an argument-less `subroutine compresid`
+ * (compresid.f90:1) five times per step from RK_S5P3 (iterations.f90:183-209).
+ * This ABI replaces that call.  The host (Fortran through ISO_C_BINDING, or
+ * any C caller) hands over the arrays that `setup2d` holds after INIT_SETUP,
+ * in the reference's own Fortran layouts and 1-based index conventions, once;
+ * afterwards the state lives in HBM and each call has the per-call contract
+ * of `compresid`.
+ *
+ * Conventions
+ *   - N is the number of solution points per direction (p+1), 3 <= N <= 5.
+ *   - Every array pointer is a plain host pointer to column-major (Fortran)
+ *     data; integer tables are 1-based exactly as the reference stores them.
+ *   - Every entry point returns SD2D_OK (0) or one of the SD2D_E* codes;
+ *     sd2d_last_error() gives a message.  Nothing exits or throws.
+ *   - One context per GPU.  Calls on one context are not re-entrant; all
+ *     device work runs on the context's own HIP stream and every call
+ *     returns after that work has completed.
+ */
+#ifndef SD2D_H
+#define SD2D_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  SD2D_OK = 0,
+  SD2D_EINVAL = 1,       /* bad argument, inconsistent mesh table          */
+  SD2D_EHIP = 2,         /* a HIP runtime call failed                      */
+  SD2D_ENOMEM = 3,       /* device or host allocation failed               */
+  SD2D_ENONFINITE = 4,   /* opt-in finiteness check found a NaN/Inf        */
+  SD2D_EUNSUPPORTED = 5  /* N or mode outside what the kernels implement   */
+};
+
+/* Opaque per-GPU context. */
+typedef struct sd2d_ctx sd2d_ctx;
+
+/* Precision of the device path. */
+enum { SD2D_FP64 = 0, SD2D_MIXED = 1 };
+
+/*
+ * Mesh, operators and metrics exactly as module setup2d holds them after
+ * INIT_SETUP (initsetup.f90:1-169) on one rank.
+ *   if2c      IF2C(NFACE,4)       connectivity.f90:446-506 (left cell, right
+ *                                 cell, left local face, right local face)
+ *   ibf*      IBFINTER/IBFINL/IBFOUT/IBFWAL/IBFSYMP/IBFREE face lists
+ *                                 (connect_bdry.f90:114-244)
+ *   cyc2fpair CYC2FPAIR(NPAIR,2)  (map_cyc_loc.f90:363-385)
+ *   lmat,mmat Lmat(N+1,N), Mmat(N+1,N) (initsetup.f90:262-319)
+ *   xs,xs_low Xs(N), Xs_low(N-1)  (initsetup.f90:171-259, :43-44)
+ *   jac,s1,s2 Jac(N,N,NCELL), S1(2,2,N+1,N,NCELL), S2(2,2,N,N+1,NCELL)
+ *                                 (calcjacob.f90:55,85-89,121-125)
+ * The arrays are copied; the caller may free them after sd2d_create.
+ */
+typedef struct sd2d_mesh {
+  int32_t N;
+  int32_t ncell;        /* NCELL (== NCELLGLOB on one rank)                 */
+  int32_t nface;        /* NFACE                                            */
+  const int32_t *if2c;
+  int32_t ninter;  const int32_t *ibfinter;
+  int32_t ninlet;  const int32_t *ibfinl;
+  int32_t noutlet; const int32_t *ibfout;
+  int32_t nwall;   const int32_t *ibfwal;
+  int32_t nsymp;   const int32_t *ibfsymp;
+  int32_t nfree;   const int32_t *ibfree;
+  int32_t npair;   const int32_t *cyc2fpair;
+  const double *lmat;
+  const double *mmat;
+  const double *xs;
+  const double *xs_low;
+  const double *jac;
+  const double *s1;
+  const double *s2;
+} sd2d_mesh;
+
+/* Run parameters: the QUAD.INP records the path reads (readinput.f90:6-43)
+ * plus the hard-coded GLM alpha (initsetup.f90:165). */
+typedef struct sd2d_params {
+  int32_t vismode;      /* 0 or 1 (bcflux.f90:207-227 wall ghost)          */
+  int32_t artiv;        /* 1: evaluate the AV smoothness sensor (AV.f90)   */
+  double mu, eta0;      /* viscosity, resistivity                          */
+  double alpha;         /* GLM damping, 2.0 in the reference               */
+  double rinf, pinf, uinf, vinf, bxinf, byinf;  /* free stream             */
+  double dxcycl, dycycl;                        /* periodic lengths        */
+  int32_t ncellglob;    /* NCELLGLOB, used by the AV length scale          */
+  int32_t k_stage;      /* 3: RK_S5P3, 5: FiveStageRk (iterations.f90:22)  */
+  double dt;            /* fixed time step                                 */
+  int32_t precision;    /* SD2D_FP64 or SD2D_MIXED                         */
+} sd2d_params;
+
+/* Running maxima of the GLM wave speed (F1 in the path description):
+ * eigvmax/umax are the local running maxima, eigvmax_g/umax_g the values
+ * after the last MPI_ALLREDUCE (compflux.f90:10-11).  They are part of the
+ * state: c_h of the next call is sqrt(eigvmax_g*(eigvmax_g-umax_g)). */
+typedef struct sd2d_maxima {
+  double eigvmax, umax, eigvmax_g, umax_g;
+} sd2d_maxima;
+
+/* Diagnostics after the most recent residual evaluation. */
+typedef struct sd2d_diag {
+  double resid_rms;     /* sqrt(sum resid^2 /(NCELLGLOB*N*N*9))  iterations.f90:61 */
+  double divb_rms;      /* sqrt(sum divB^2  /(NCELLGLOB*N*N))    iterations.f90:62 */
+  double eigvmax_g, umax_g;
+  double c_h;           /* c_h used by the last residual evaluation        */
+  double epsilon0;      /* AV.f90:126-127                                  */
+  double ctime;
+  int32_t iter;
+  int32_t norms_valid;  /* 1 if resid_rms/divb_rms belong to the last step */
+} sd2d_diag;
+
+/* Create a context on the current HIP device.  Replaces the allocation and
+ * set-up of the path's module arrays in INIT_SETUP (initsetup.f90:12-80). */
+int sd2d_create(const sd2d_mesh *mesh, const sd2d_params *params,
+                sd2d_ctx **out);
+
+/* Free every device and host resource of the context. */
+void sd2d_destroy(sd2d_ctx *ctx);
+
+/* Message for the last error on this context (or a global one if ctx is
+ * NULL); never NULL. */
+const char *sd2d_last_error(const sd2d_ctx *ctx);
+
+/* Upload Q(9,N,N,NCELL) (setup2d.f90:54) and, if not NULL, the running
+ * maxima.  Replaces the IC/restart load of INIT_SETUP
+ * (initsetup.f90:154-167). */
+int sd2d_set_state(sd2d_ctx *ctx, const double *q,
+                   const sd2d_maxima *maxima);
+
+/* Download Q(9,N,N,NCELL) and/or the running maxima (either may be NULL). */
+int sd2d_get_state(sd2d_ctx *ctx, double *q, sd2d_maxima *maxima);
+
+/* Simulation clock carried with the state (setup2d.f90:78-79). */
+int sd2d_set_time(sd2d_ctx *ctx, double ctime, int32_t iter);
+
+/*
+ * The per-call contract of `compresid` (compresid.f90:1-80, with
+ * compflux.f90:1-33 and everything it calls): from the device state Q and
+ * the lagged maxima, compute resid(9,N,N,NCELL) and
+ * nablaQs(9,2,N,N,NCELL) and update the running maxima and the AV sensor.
+ * resid_out / nablaqs_out are host pointers in Fortran layout; either may
+ * be NULL, in which case that result is not copied back.
+ */
+int sd2d_compresid(sd2d_ctx *ctx, double *resid_out,
+                   double *nablaqs_out);
+
+/* Convenience for a Fortran host that keeps Q in its own module array:
+ * sd2d_set_state(ctx, q, NULL) followed by sd2d_compresid. */
+int sd2d_compresid_q(sd2d_ctx *ctx, const double *q,
+                     double *resid_out, double *nablaqs_out);
+
+/*
+ * Device-resident time stepping: `nsteps` calls of RK_S5P3
+ * (iterations.f90:158-213) or FiveStageRk (iterations.f90:88-156) as chosen
+ * by params.k_stage, including the ctime bookkeeping.  When norm_every > 0,
+ * the residual and divB RMS norms of iterations.f90:34-62 are evaluated on
+ * the steps with mod(iter, norm_every) == 1 and the latest is available
+ * from sd2d_get_diag.
+ */
+int sd2d_step(sd2d_ctx *ctx, int32_t nsteps, int32_t norm_every);
+
+/* Diagnostics of the latest stage / step. */
+int sd2d_get_diag(sd2d_ctx *ctx, sd2d_diag *diag);
+
+/*
+ * AV sensor output of the latest residual evaluation:
+ * SM_IND(2,NCELL) and muAV(2,NCELL) (AV.f90:83,129-140) and epsilon0
+ * (AV.f90:126-127).  Any pointer may be NULL.  Requires params.artiv == 1.
+ */
+int sd2d_get_av(sd2d_ctx *ctx, double *sm_ind, double *muav,
+                double *epsilon0);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SD2D_H */

@@ -1,0 +1,1135 @@
+/*
+ * sd2d_oracle.c - CPU restatement of the reference path (test
+ * infrastructure: the checker and the CPU baseline, never the product).
+ *
+ * Each routine cites the reference routine it restates.  Arrays keep the
+ * reference's Fortran layouts; the index macros below map a 0-based
+ * (Fortran-order) multi-index to the flat offset.  Expressions keep the
+ * reference's operation order (Fortran evaluates a*b*c as (a*b)*c and
+ * a+b+c as (a+b)+c) and the file is compiled with -ffp-contract=off, so the
+ * results equal those of the reference built without FMA contraction.
+ *
+ * The only deliberate difference: cell and face loops are distributed over
+ * OpenMP threads.  Every output element has exactly one writer and the
+ * running maxima are combined with max, which is exact and order
+ * independent, so the result does not depend on the thread count.  The
+ * residual norms are summed serially in the reference's order.
+ */
+#include "sd2d_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define GAM (5.0 / 3)        /* setup2d.f90:67 */
+#define LAMBDA (-2.0 / 3.0)  /* setup2d.f90:69 */
+#define PRANDT 0.72          /* setup2d.f90:70 */
+static const double PI_ = 3.141592653589793; /* atan(1d0)*4d0, setup2d.f90:66 */
+
+struct orc_ctx {
+  int N, nc, nface;
+  int *if2c;
+  int ninter, ninlet, noutlet, nwall, nsymp, nfree, npair;
+  int *ibfinter, *ibfinl, *ibfout, *ibfwal, *ibfsymp, *ibfree, *cyc2fpair;
+  double *L, *Mm, *xs, *xs_low, *jac, *s1, *s2;
+  sd2d_params p;
+  int iface2fp[4 * 8], jface2fp[4 * 8];
+  /* hhval(is, Xs_low(isl)) and hhval_low(isl, Xs(is)) tables (AV.f90) */
+  double *hlow, *hhigh;
+  /* module arrays of setup2d.f90:54-63,86 */
+  double *Q, *resid, *F1, *G2, *Fv1, *Gv2, *Qvfi, *Qvfj;
+  double *nablaQs, *nablaQvfi, *nablaQvfj, *dedxi, *dedeta;
+  double *muAV, *muAVfi, *muAVfj, *SM_IND;
+  double eigvmax, umax, eigvmax_g, umax_g, c_h, epsilon0;
+  double ctime;
+  int iter;
+  double resid_rms, divb_rms;
+};
+
+/* ---- Fortran index maps (0-based arguments) ---------------------------- */
+#define NN (o->N)
+#define IQ(v, i, j, c) ((v) + 9 * ((i) + NN * ((j) + NN * (size_t)(c))))
+#define IF1(v, i, j, c) ((v) + 9 * ((i) + (NN + 1) * ((j) + NN * (size_t)(c))))
+#define IG2(v, i, j, c) ((v) + 9 * ((i) + NN * ((j) + (NN + 1) * (size_t)(c))))
+#define IS1(a, b, i, j, c) \
+  ((a) + 2 * ((b) + 2 * ((i) + (NN + 1) * ((j) + NN * (size_t)(c)))))
+#define IS2(a, b, i, j, c) \
+  ((a) + 2 * ((b) + 2 * ((i) + NN * ((j) + (NN + 1) * (size_t)(c)))))
+#define IJ(i, j, c) ((i) + NN * ((j) + NN * (size_t)(c)))
+#define INQ(v, d, i, j, c) ((v) + 9 * ((d) + 2 * ((i) + NN * ((j) + NN * (size_t)(c)))))
+#define INQI(v, d, i, j, c) \
+  ((v) + 9 * ((d) + 2 * ((i) + (NN + 1) * ((j) + NN * (size_t)(c)))))
+#define INQJ(v, d, i, j, c) \
+  ((v) + 9 * ((d) + 2 * ((i) + NN * ((j) + (NN + 1) * (size_t)(c)))))
+#define LM(ifp, is) (o->L[(ifp) + (NN + 1) * (is)])
+#define MM(ifp, is) (o->Mm[(ifp) + (NN + 1) * (is)])
+#define IF2C(f, k) (o->if2c[(f) + (size_t)o->nface * (k)]) /* f 0-based, value 1-based */
+#define MUFI(v, f, c) ((v) + 2 * ((f) + 2 * (size_t)(c)))
+
+static int *dupi(const int32_t *a, int n) {
+  int *r = (int *)malloc(sizeof(int) * (n > 0 ? n : 1));
+  if (n > 0) memcpy(r, a, sizeof(int) * n);
+  return r;
+}
+static double *dupd(const double *a, size_t n) {
+  double *r = (double *)malloc(sizeof(double) * (n > 0 ? n : 1));
+  if (n > 0 && a) memcpy(r, a, sizeof(double) * n);
+  return r;
+}
+static double *zd(size_t n) { return (double *)calloc(n > 0 ? n : 1, sizeof(double)); }
+
+int orc_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+/* hhval (initsetup.f90:322-344) on an arbitrary node set */
+static double lagr(const double *nodes, int np, int i, double x) {
+  double num = 1.0, den = 1.0;
+  for (int s = 0; s < np; s++)
+    if (s != i) {
+      num = num * (x - nodes[s]);
+      den = den * (nodes[i] - nodes[s]);
+    }
+  return num / den;
+}
+
+orc_ctx *orc_create(const sd2d_mesh *m, const sd2d_params *p) {
+  if (!m || !p || m->N < 2 || m->N > 8 || m->ncell <= 0) return NULL;
+  orc_ctx *o = (orc_ctx *)calloc(1, sizeof(orc_ctx));
+  int N = m->N, nc = m->ncell;
+  o->N = N; o->nc = nc; o->nface = m->nface; o->p = *p;
+  o->if2c = dupi(m->if2c, 4 * m->nface);
+  o->ninter = m->ninter; o->ibfinter = dupi(m->ibfinter, m->ninter);
+  o->ninlet = m->ninlet; o->ibfinl = dupi(m->ibfinl, m->ninlet);
+  o->noutlet = m->noutlet; o->ibfout = dupi(m->ibfout, m->noutlet);
+  o->nwall = m->nwall; o->ibfwal = dupi(m->ibfwal, m->nwall);
+  o->nsymp = m->nsymp; o->ibfsymp = dupi(m->ibfsymp, m->nsymp);
+  o->nfree = m->nfree; o->ibfree = dupi(m->ibfree, m->nfree);
+  o->npair = m->npair; o->cyc2fpair = dupi(m->cyc2fpair, 2 * m->npair);
+  o->L = dupd(m->lmat, (N + 1) * N);
+  o->Mm = dupd(m->mmat, (N + 1) * N);
+  o->xs = dupd(m->xs, N);
+  o->xs_low = dupd(m->xs_low, N - 1);
+  o->jac = dupd(m->jac, (size_t)N * N * nc);
+  o->s1 = dupd(m->s1, (size_t)4 * (N + 1) * N * nc);
+  o->s2 = dupd(m->s2, (size_t)4 * N * (N + 1) * nc);
+  /* face2fpconnec (initsetup.f90:347-383), 1-based values */
+  for (int nfp = 1; nfp <= N; nfp++) {
+    o->iface2fp[(nfp - 1) + N * 0] = nfp;          o->jface2fp[(nfp - 1) + N * 0] = 1;
+    o->iface2fp[(nfp - 1) + N * 1] = N + 1;        o->jface2fp[(nfp - 1) + N * 1] = nfp;
+    o->iface2fp[(nfp - 1) + N * 2] = N - nfp + 1;  o->jface2fp[(nfp - 1) + N * 2] = N + 1;
+    o->iface2fp[(nfp - 1) + N * 3] = 1;            o->jface2fp[(nfp - 1) + N * 3] = N - nfp + 1;
+  }
+  o->hlow = zd((size_t)N * (N - 1));
+  o->hhigh = zd((size_t)N * (N - 1));
+  for (int a = 0; a < N - 1; a++)
+    for (int i = 0; i < N; i++) {
+      o->hlow[i + N * a] = lagr(o->xs, N, i, o->xs_low[a]);       /* hhval(is,xsl) */
+      o->hhigh[a + (N - 1) * i] = lagr(o->xs_low, N - 1, a, o->xs[i]); /* hhval_low */
+    }
+  size_t nq = (size_t)9 * N * N * nc, nf = (size_t)9 * (N + 1) * N * nc;
+  o->Q = zd(nq); o->resid = zd(nq);
+  o->F1 = zd(nf); o->G2 = zd(nf); o->Fv1 = zd(nf); o->Gv2 = zd(nf);
+  o->Qvfi = zd(nf); o->Qvfj = zd(nf);
+  o->nablaQs = zd(2 * nq); o->nablaQvfi = zd(2 * nf); o->nablaQvfj = zd(2 * nf);
+  o->dedxi = zd((size_t)N * N * nc); o->dedeta = zd((size_t)N * N * nc);
+  o->muAV = zd(2 * (size_t)nc); o->SM_IND = zd(2 * (size_t)nc);
+  o->muAVfi = zd(4 * (size_t)nc); o->muAVfj = zd(4 * (size_t)nc);
+  /* initsetup.f90:165-167 and the zero default of the _g copies */
+  o->eigvmax = o->umax = o->eigvmax_g = o->umax_g = o->c_h = 0.0;
+  return o;
+}
+
+void orc_destroy(orc_ctx *o) {
+  if (!o) return;
+  void *ptrs[] = {o->if2c, o->ibfinter, o->ibfinl, o->ibfout, o->ibfwal,
+                  o->ibfsymp, o->ibfree, o->cyc2fpair, o->L, o->Mm, o->xs,
+                  o->xs_low, o->jac, o->s1, o->s2, o->hlow, o->hhigh, o->Q,
+                  o->resid, o->F1, o->G2, o->Fv1, o->Gv2, o->Qvfi, o->Qvfj,
+                  o->nablaQs, o->nablaQvfi, o->nablaQvfj, o->dedxi, o->dedeta,
+                  o->muAV, o->SM_IND, o->muAVfi, o->muAVfj};
+  for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) free(ptrs[i]);
+  free(o);
+}
+
+int orc_set_state(orc_ctx *o, const double *q, const sd2d_maxima *mx) {
+  if (q) memcpy(o->Q, q, sizeof(double) * 9 * o->N * o->N * (size_t)o->nc);
+  if (mx) {
+    o->eigvmax = mx->eigvmax; o->umax = mx->umax;
+    o->eigvmax_g = mx->eigvmax_g; o->umax_g = mx->umax_g;
+  }
+  return 0;
+}
+
+int orc_get_state(orc_ctx *o, double *q, sd2d_maxima *mx) {
+  if (q) memcpy(q, o->Q, sizeof(double) * 9 * o->N * o->N * (size_t)o->nc);
+  if (mx) {
+    mx->eigvmax = o->eigvmax; mx->umax = o->umax;
+    mx->eigvmax_g = o->eigvmax_g; mx->umax_g = o->umax_g;
+  }
+  return 0;
+}
+
+void orc_set_time(orc_ctx *o, double ctime, int iter) { o->ctime = ctime; o->iter = iter; }
+void orc_get_time(orc_ctx *o, double *ctime, int *iter) {
+  if (ctime) *ctime = o->ctime;
+  if (iter) *iter = o->iter;
+}
+
+/* ---- pointwise physics (getrusanov.f90) --------------------------------- */
+
+/* getIfluxvectors (getrusanov.f90:127-193).  *umax is a thread-local
+ * running maximum. */
+static void getIfluxvectors(const double *Qi, double *Fi, double *Gi, double c_h,
+                            double *umax) {
+  double rho = Qi[0], rhou = Qi[1], rhov = Qi[2], rhow = Qi[3], rhoE = Qi[4];
+  double Bx = Qi[5], By = Qi[6], Bz = Qi[7], psi = Qi[8];
+  double NORM_B = Bx * Bx + By * By + Bz * Bz;
+  double pr = (GAM - 1.0) * (((rhoE - 0.5 * (rhou * rhou + rhov * rhov + rhow * rhow) / rho)
+                              - 0.5 * NORM_B) - 0.5 * (psi * psi));
+  Fi[0] = rhou;
+  Fi[1] = ((rhou * rhou) / rho + pr) + (0.5 * NORM_B - Bx * Bx);
+  Fi[2] = rhou * rhov / rho - Bx * By;
+  Fi[3] = rhou * rhow / rho - Bx * Bz;
+  Fi[4] = (rhou / rho) * (((rhoE - 0.5 * (psi * psi)) + pr) + 0.5 * NORM_B)
+          - ((rhou / rho * Bx + rhov / rho * By) + rhow / rho * Bz) * Bx;
+  Fi[5] = 0.0;
+  Fi[6] = rhou / rho * By - rhov / rho * Bx;
+  Fi[7] = rhou / rho * Bz - rhow / rho * Bx;
+  Gi[0] = rhov;
+  Gi[1] = rhou * rhov / rho - Bx * By;
+  Gi[2] = ((rhov * rhov) / rho + pr) + (0.5 * NORM_B - By * By);
+  Gi[3] = rhow * rhov / rho - By * Bz;
+  Gi[4] = (rhov / rho) * (((rhoE - 0.5 * (psi * psi)) + pr) + 0.5 * NORM_B)
+          - ((rhou / rho * Bx + rhov / rho * By) + rhow / rho * Bz) * By;
+  Gi[5] = rhov / rho * Bx - rhou / rho * By;
+  Gi[6] = 0.0;
+  Gi[7] = rhov / rho * Bz - rhow / rho * By;
+  /* divergence cleaning part (getrusanov.f90:174-181) */
+  Fi[4] = Fi[4] + c_h * psi * Bx;
+  Gi[4] = Gi[4] + c_h * psi * By;
+  Fi[5] = Fi[5] + c_h * psi;
+  Gi[6] = Gi[6] + c_h * psi;
+  Fi[8] = c_h * Bx;
+  Gi[8] = c_h * By;
+  double u = rhou / rho, v = rhov / rho, w = rhow / rho;
+  double ut = fmax(fmax(fabs(u), fabs(v)), fabs(w));
+  if (ut > *umax) *umax = ut;
+}
+
+/* getrusanovflux (getrusanov.f90:1-125) */
+static void getrusanovflux(const double *Qfl, const double *Qfr, double *Fnl, double *Fnr,
+                           const double *normf, int sign_l, int sign_r, double c_h,
+                           double *eigvmax, double *umax) {
+  double magnorm = sqrt(normf[0] * normf[0] + normf[1] * normf[1]);
+  double unit_x = normf[0] / magnorm * sign_l;
+  double unit_y = normf[1] / magnorm * sign_l;
+  double rhoi = 1.0 / Qfl[0];
+  double ul = rhoi * Qfl[1], vl = rhoi * Qfl[2], wl = rhoi * Qfl[3];
+  double Bxl = Qfl[5], Byl = Qfl[6], Bzl = Qfl[7], psil = Qfl[8];
+  double NORM_Bl = Bxl * Bxl + Byl * Byl + Bzl * Bzl;
+  double pl = (GAM - 1.0) * (((Qfl[4] - 0.5 * Qfl[0] * (ul * ul + vl * vl + wl * wl))
+                              - NORM_Bl * 0.5) - 0.5 * (psil * psil));
+  pl = pl + 0.5 * NORM_Bl;
+  double Vn_l = ul * unit_x + vl * unit_y;
+  double Bn_l = Bxl * unit_x + Byl * unit_y;
+  rhoi = 1.0 / Qfr[0];
+  double ur = rhoi * Qfr[1], vr = rhoi * Qfr[2], wr = rhoi * Qfr[3];
+  double Bxr = Qfr[5], Byr = Qfr[6], Bzr = Qfr[7], psir = Qfr[8];
+  double NORM_Br = Bxr * Bxr + Byr * Byr + Bzr * Bzr;
+  double pr = (GAM - 1.0) * (((Qfr[4] - 0.5 * Qfr[0] * (ur * ur + vr * vr + wr * wr))
+                              - NORM_Br * 0.5) - 0.5 * (psir * psir));
+  pr = pr + NORM_Br * 0.5;
+  double Vn_r = ur * unit_x + vr * unit_y;
+  double Bn_r = Bxr * unit_x + Byr * unit_y;
+
+  Fnl[0] = Qfl[0] * Vn_l;
+  Fnl[1] = (Qfl[1] * Vn_l + pl * unit_x) - Bxl * Bn_l;
+  Fnl[2] = (Qfl[2] * Vn_l + pl * unit_y) - Byl * Bn_l;
+  Fnl[3] = Qfl[3] * Vn_l - Bzl * Bn_l;
+  Fnl[4] = ((Qfl[4] - 0.5 * (psil * psil)) + pl) * Vn_l - Bn_l * ((ul * Bxl + vl * Byl) + wl * Bzl);
+  Fnl[5] = (vl * Bxl - ul * Byl) * unit_y;
+  Fnl[6] = (ul * Byl - vl * Bxl) * unit_x;
+  Fnl[7] = Bzl * Vn_l - wl * Bn_l;
+
+  Fnr[0] = Qfr[0] * Vn_r;
+  Fnr[1] = (Qfr[1] * Vn_r + pr * unit_x) - Bxr * Bn_r;
+  Fnr[2] = (Qfr[2] * Vn_r + pr * unit_y) - Byr * Bn_r;
+  Fnr[3] = Qfr[3] * Vn_r - Bzr * Bn_r;
+  Fnr[4] = ((Qfr[4] - 0.5 * (psir * psir)) + pr) * Vn_r - Bn_r * ((ur * Bxr + vr * Byr) + wr * Bzr);
+  Fnr[5] = (vr * Bxr - ur * Byr) * unit_y;
+  Fnr[6] = (ur * Byr - vr * Bxr) * unit_x;
+  Fnr[7] = Bzr * Vn_r - wr * Bn_r;
+
+  /* divergence cleaning part (getrusanov.f90:78-88) */
+  Fnl[4] = Fnl[4] + c_h * psil * Bn_l;
+  Fnr[4] = Fnr[4] + c_h * psir * Bn_r;
+  Fnl[5] = Fnl[5] + c_h * psil * unit_x;
+  Fnr[5] = Fnr[5] + c_h * psir * unit_x;
+  Fnl[6] = Fnl[6] + c_h * psil * unit_y;
+  Fnr[6] = Fnr[6] + c_h * psir * unit_y;
+  Fnl[8] = c_h * Bn_l;
+  Fnr[8] = c_h * Bn_r;
+
+  /* fast magneto-acoustic speed (getrusanov.f90:92-105) */
+  pl = pl - 0.5 * NORM_Bl;
+  pr = pr - 0.5 * NORM_Br;
+  double Bx = 0.5 * (Bxl + Bxr), By = 0.5 * (Byl + Byr), Bz = 0.5 * (Bzl + Bzr);
+  double c_a = GAM * (pl + pr) / (Qfl[0] + Qfr[0]);
+  double c_d = ((Bx * Bx + By * By) + Bz * Bz) / (0.5 * (Qfl[0] + Qfr[0]));
+  double bn = Bx * unit_x + By * unit_y;
+  double c_n = (bn * bn) / (0.5 * (Qfl[0] + Qfr[0]));
+  double cad = c_a + c_d;
+  double c_tmp = sqrt(cad * cad - 4.0 * c_a * c_n);
+  double c_f = sqrt(0.5 * ((c_a + c_d) + c_tmp));
+  double eigv = 0.5 * fabs(Vn_l + Vn_r) + c_f;
+  for (int k = 0; k < 9; k++) Fnl[k] = 0.5 * ((Fnl[k] + Fnr[k]) - eigv * (Qfr[k] - Qfl[k]));
+  for (int k = 0; k < 9; k++) Fnl[k] = magnorm * sign_l * Fnl[k];
+  for (int k = 0; k < 9; k++) Fnr[k] = (double)(sign_l * sign_r) * Fnl[k];
+  if (eigv > *eigvmax) *eigvmax = eigv;
+  double ut = fmax(fmax(0.5 * fabs(ul + ur), 0.5 * fabs(vl + vr)), 0.5 * fabs(wl + wr));
+  if (ut > *umax) *umax = ut;
+}
+
+/* getVfluxvectors (getrusanov.f90:197-294) with viscosity mu_ and
+ * resistivity eta_; getVfluxvectors_AV (:296-413) is the same formula
+ * with the hard-coded mu_art = eta_art = 0 (:325-328). */
+static void getVflux(const double *Qv, const double *nq /* (9,2) */, double mu_, double eta_,
+                     double *Fv, double *Gv) {
+  double rho = Qv[0], rhou = Qv[1], rhov = Qv[2], rhow = Qv[3], rhoE = Qv[4];
+  double Bx = Qv[5], By = Qv[6], Bz = Qv[7], psi = Qv[8];
+  double u = rhou / rho, v = rhov / rho, w = rhow / rho;
+  double NORM_U = (u * u + v * v) + w * w;
+  double NORM_B = (Bx * Bx + By * By) + Bz * Bz;
+  double inte = ((((rhoE - 0.5 * NORM_U * rho) - 0.5 * NORM_B) - 0.5 * (psi * psi))) / rho;
+  double drho[2], du[2], dv[2], dw[2], dBx[2], dBy[2], dBz[2], dpsi[2], dKE[2], dpre_B[2],
+      dpsi2[2], de[2];
+  for (int d = 0; d < 2; d++) {
+    const double *g = nq + 9 * d;
+    drho[d] = g[0];
+    du[d] = (g[1] - drho[d] * u) / rho;
+    dv[d] = (g[2] - drho[d] * v) / rho;
+    dw[d] = (g[3] - drho[d] * w) / rho;
+    dBx[d] = g[5]; dBy[d] = g[6]; dBz[d] = g[7]; dpsi[d] = g[8];
+  }
+  for (int d = 0; d < 2; d++) {
+    dKE[d] = 0.5 * NORM_U * drho[d] + rho * ((u * du[d] + v * dv[d]) + w * dw[d]);
+    dpre_B[d] = (Bx * dBx[d] + By * dBy[d]) + Bz * dBz[d];
+    dpsi2[d] = psi * dpsi[d];
+    de[d] = ((((nq[4 + 9 * d] - dKE[d]) - dpre_B[d]) - drho[d] * inte) - dpsi2[d]) / rho;
+  }
+  double ux = du[0], uy = du[1], vx = dv[0], vy = dv[1], wx = dw[0], wy = dw[1];
+  double ex = de[0], ey = de[1];
+  double Bxy = dBx[1], Byx = dBy[0], Bzx = dBz[0], Bzy = dBz[1];
+  Fv[0] = 0.0;
+  Fv[1] = mu_ * (2.0 * ux + LAMBDA * (ux + vy));
+  Fv[2] = mu_ * (vx + uy);
+  Fv[3] = mu_ * wx;
+  Fv[4] = ((u * Fv[1] + v * Fv[2]) + w * Fv[3]) + mu_ * GAM * ex / PRANDT;
+  Gv[0] = 0.0;
+  Gv[1] = mu_ * (vx + uy);
+  Gv[2] = mu_ * (2.0 * vy + LAMBDA * (ux + vy));
+  Gv[3] = mu_ * wy;
+  Gv[4] = ((Gv[1] * u + Gv[2] * v) + Gv[3] * w) + mu_ * GAM * ey / PRANDT;
+  double Jx = Bzy - 0.0, Jy = 0.0 - Bzx, Jz = Byx - Bxy;
+  Fv[5] = 0.0;
+  Fv[6] = eta_ * Jz;
+  Fv[7] = -eta_ * Jy;
+  Fv[4] = (Fv[4] + Fv[6] * By) + Fv[7] * Bz;
+  Gv[5] = -eta_ * Jz;
+  Gv[6] = 0.0;
+  Gv[7] = eta_ * Jx;
+  Gv[4] = (Gv[4] + Gv[5] * Bx) + Gv[7] * Bz;
+  Fv[8] = 0.0;
+  Gv[8] = 0.0;
+}
+
+/* ---- traces ------------------------------------------------------------ */
+
+/* Qf = sum_sp Q(:,sp,jfp,ic)*Lmat(ifp,sp) for fam 1, or
+ *      sum_sp Q(:,ifp,sp,ic)*Lmat(jfp,sp) for fam 2 (1-based ifp/jfp)
+ * (interfaceflux_all.f90:40-55). */
+static void trace_q(const orc_ctx *o, const double *A, int ic, int fam, int ifp, int jfp,
+                    double *Qf) {
+  for (int v = 0; v < 9; v++) Qf[v] = 0.0;
+  for (int sp = 0; sp < NN; sp++) {
+    const double *q = (fam == 1) ? &A[IQ(0, sp, jfp - 1, ic)] : &A[IQ(0, ifp - 1, sp, ic)];
+    double l = (fam == 1) ? LM(ifp - 1, sp) : LM(jfp - 1, sp);
+    for (int v = 0; v < 9; v++) Qf[v] = Qf[v] + q[v] * l;
+  }
+}
+
+/* Same for both gradient directions of nablaQs (interfaceflux_all.f90:140-152) */
+static void trace_grad(const orc_ctx *o, int ic, int fam, int ifp, int jfp, double *G1,
+                       double *G2_) {
+  for (int v = 0; v < 9; v++) { G1[v] = 0.0; G2_[v] = 0.0; }
+  for (int sp = 0; sp < NN; sp++) {
+    const double *g = (fam == 1) ? &o->nablaQs[INQ(0, 0, sp, jfp - 1, ic)]
+                                 : &o->nablaQs[INQ(0, 0, ifp - 1, sp, ic)];
+    double l = (fam == 1) ? LM(ifp - 1, sp) : LM(jfp - 1, sp);
+    for (int v = 0; v < 9; v++) {
+      G1[v] = G1[v] + g[v] * l;
+      G2_[v] = G2_[v] + g[9 + v] * l;
+    }
+  }
+}
+
+static double *fluxslot(orc_ctx *o, double *F, double *G, int fam, int ifp, int jfp, int ic) {
+  return (fam == 1) ? &F[IF1(0, ifp - 1, jfp - 1, ic)] : &G[IG2(0, ifp - 1, jfp - 1, ic)];
+}
+
+static void normal_of(const orc_ctx *o, int fam, int ifp, int jfp, int ic, double *nf) {
+  if (fam == 1) {
+    nf[0] = o->s1[IS1(0, 0, ifp - 1, jfp - 1, ic)];
+    nf[1] = o->s1[IS1(0, 1, ifp - 1, jfp - 1, ic)];
+  } else {
+    nf[0] = o->s2[IS2(1, 0, ifp - 1, jfp - 1, ic)];
+    nf[1] = o->s2[IS2(1, 1, ifp - 1, jfp - 1, ic)];
+  }
+}
+
+/* ---- inviscid fluxes ---------------------------------------------------- */
+
+/* inviscid_flux_interior (compflux.f90:53-104) */
+static void inviscid_flux_interior(orc_ctx *o, double *umax) {
+  const int N = o->N;
+  double um = *umax;
+#pragma omp parallel for reduction(max : um) schedule(static)
+  for (int ic = 0; ic < o->nc; ic++) {
+    double Qf[9], F[9], G[9];
+    for (int jfp = 0; jfp < N; jfp++)
+      for (int ifp = 1; ifp < N; ifp++) {
+        trace_q(o, o->Q, ic, 1, ifp + 1, jfp + 1, Qf);
+        if (o->p.vismode == 1)
+          for (int v = 0; v < 9; v++) o->Qvfi[IF1(v, ifp, jfp, ic)] = Qf[v];
+        getIfluxvectors(Qf, F, G, o->c_h, &um);
+        double a = o->s1[IS1(0, 0, ifp, jfp, ic)], b = o->s1[IS1(0, 1, ifp, jfp, ic)];
+        for (int v = 0; v < 9; v++) o->F1[IF1(v, ifp, jfp, ic)] = F[v] * a + G[v] * b;
+      }
+    for (int jfp = 1; jfp < N; jfp++)
+      for (int ifp = 0; ifp < N; ifp++) {
+        trace_q(o, o->Q, ic, 2, ifp + 1, jfp + 1, Qf);
+        if (o->p.vismode == 1)
+          for (int v = 0; v < 9; v++) o->Qvfj[IG2(v, ifp, jfp, ic)] = Qf[v];
+        getIfluxvectors(Qf, F, G, o->c_h, &um);
+        double a = o->s2[IS2(1, 0, ifp, jfp, ic)], b = o->s2[IS2(1, 1, ifp, jfp, ic)];
+        for (int v = 0; v < 9; v++) o->G2[IG2(v, ifp, jfp, ic)] = F[v] * a + G[v] * b;
+      }
+  }
+  *umax = um;
+}
+
+/* One face with two cells: interfaceflux (interfaceflux_all.f90:16-110)
+ * and CYCLOCFLUX (cyclocflux_all.f90:15-305) differ only in how the cells
+ * and local faces are looked up. */
+static void two_cell_face(orc_ctx *o, int icleft, int icright, int ifacelc, int ifacerc,
+                          double *eig, double *um) {
+  const int N = o->N;
+  int faml = ifacelc % 2 + 1, famr = ifacerc % 2 + 1;
+  for (int nfp = 1; nfp <= N; nfp++) {
+    double Qfl[9], Qfr[9], Fnl[9], Fnr[9], normf[2];
+    int ifpl = o->iface2fp[(nfp - 1) + N * (ifacelc - 1)];
+    int jfpl = o->jface2fp[(nfp - 1) + N * (ifacelc - 1)];
+    int sign_l = 1, sign_r = 1;
+    trace_q(o, o->Q, icleft, faml, ifpl, jfpl, Qfl);
+    if (faml == 1) { if (ifpl == 1) sign_l = -1; }
+    else { if (jfpl == 1) sign_l = -1; }
+    normal_of(o, faml, ifpl, jfpl, icleft, normf);
+    int ifpr = o->iface2fp[(N - nfp) + N * (ifacerc - 1)];
+    int jfpr = o->jface2fp[(N - nfp) + N * (ifacerc - 1)];
+    trace_q(o, o->Q, icright, famr, ifpr, jfpr, Qfr);
+    if (famr == 1) { if (ifpr == N + 1) sign_r = -1; }
+    else { if (jfpr == N + 1) sign_r = -1; }
+    if (o->p.vismode == 1) {
+      double *a = fluxslot(o, o->Qvfi, o->Qvfj, faml, ifpl, jfpl, icleft);
+      double *b = fluxslot(o, o->Qvfi, o->Qvfj, famr, ifpr, jfpr, icright);
+      for (int v = 0; v < 9; v++) {
+        double av = 0.5 * Qfl[v] + 0.5 * Qfr[v];
+        a[v] = av; b[v] = av;
+      }
+    }
+    getrusanovflux(Qfl, Qfr, Fnl, Fnr, normf, sign_l, sign_r, o->c_h, eig, um);
+    double *fl = fluxslot(o, o->F1, o->G2, faml, ifpl, jfpl, icleft);
+    double *fr = fluxslot(o, o->F1, o->G2, famr, ifpr, jfpr, icright);
+    for (int v = 0; v < 9; v++) { fl[v] = Fnl[v]; fr[v] = Fnr[v]; }
+  }
+}
+
+static void interfaceflux(orc_ctx *o, double *eigvmax, double *umax) {
+  double eig = *eigvmax, um = *umax;
+#pragma omp parallel for reduction(max : eig, um) schedule(static)
+  for (int f = 0; f < o->ninter; f++) {
+    int iface = o->ibfinter[f] - 1;
+    two_cell_face(o, IF2C(iface, 0) - 1, IF2C(iface, 1) - 1, IF2C(iface, 2), IF2C(iface, 3),
+                  &eig, &um);
+  }
+  *eigvmax = eig; *umax = um;
+}
+
+static void cyclocflux(orc_ctx *o, double *eigvmax, double *umax) {
+  double eig = *eigvmax, um = *umax;
+#pragma omp parallel for reduction(max : eig, um) schedule(static)
+  for (int ip = 0; ip < o->npair; ip++) {
+    int K1 = o->cyc2fpair[ip] - 1, K2 = o->cyc2fpair[ip + o->npair] - 1;
+    two_cell_face(o, IF2C(K1, 0) - 1, IF2C(K2, 0) - 1, IF2C(K1, 2), IF2C(K2, 2), &eig, &um);
+  }
+  *eigvmax = eig; *umax = um;
+}
+
+enum { BC_INLE = 1, BC_OUTL, BC_WALL, BC_SYMP, BC_FREE };
+
+/* Ghost state of BCFLUX (bcflux.f90:17-375) */
+static void bc_ghost(const orc_ctx *o, int type, const double *Qfl, const double *normf,
+                     double *Qfr) {
+  const sd2d_params *p = &o->p;
+  double rhor, ur, vr, wr, Bxr, Byr, Bzr, rhow, uw, vw, ww, magnorm, VN;
+  switch (type) {
+  case BC_INLE: /* bcflux.f90:51-72 */
+    rhor = p->rinf; ur = p->uinf; vr = p->vinf; wr = 0.0;
+    Bxr = p->bxinf; Byr = p->byinf; Bzr = 0.0;
+    Qfr[0] = rhor; Qfr[1] = rhor * ur; Qfr[2] = rhor * vr; Qfr[3] = rhor * wr;
+    Qfr[4] = ((p->pinf / (GAM - 1) + 0.5 * rhor * ((ur * ur + vr * vr) + wr * wr))
+              + 0.5 * ((Bxr * Bxr + Byr * Byr) + Bzr * Bzr)) + 0.5 * (Qfl[8] * Qfl[8]);
+    Qfr[5] = Bxr; Qfr[6] = Byr; Qfr[7] = Bzr; Qfr[8] = Qfl[8];
+    break;
+  case BC_OUTL: /* bcflux.f90:122-143 */
+    rhor = Qfl[0]; ur = Qfl[1] / Qfl[0]; vr = Qfl[2] / Qfl[0]; wr = 0.0;
+    Bxr = p->bxinf; Byr = p->byinf; Bzr = 0.0;
+    Qfr[0] = rhor; Qfr[1] = rhor * ur; Qfr[2] = rhor * vr; Qfr[3] = rhor * wr;
+    Qfr[4] = ((p->pinf / (GAM - 1) + 0.5 * rhor * ((ur * ur + vr * vr) + wr * wr))
+              + 0.5 * ((Bxr * Bxr + Byr * Byr) + Bzr * Bzr)) + 0.5 * (Qfl[8] * Qfl[8]);
+    Qfr[5] = Bxr; Qfr[6] = Byr; Qfr[7] = Bzr; Qfr[8] = 0.0;
+    break;
+  case BC_WALL: /* bcflux.f90:196-227 */
+    rhow = Qfl[0]; uw = Qfl[1] / rhow; vw = Qfl[2] / rhow; ww = Qfl[3] / rhow;
+    Qfr[0] = rhow; Qfr[4] = Qfl[4];
+    Qfr[5] = -Qfl[5]; Qfr[6] = -Qfl[6]; Qfr[7] = -Qfl[7]; Qfr[8] = Qfl[8];
+    if (p->vismode == 0) {
+      magnorm = sqrt(normf[0] * normf[0] + normf[1] * normf[1]);
+      VN = uw * normf[0] / magnorm + vw * normf[1] / magnorm;
+      Qfr[1] = rhow * (uw - 2.0 * VN * normf[0] / magnorm);
+      Qfr[2] = rhow * (vw - 2.0 * VN * normf[1] / magnorm);
+      Qfr[3] = -rhow * ww;
+    } else {
+      Qfr[1] = -rhow * uw; Qfr[2] = -rhow * vw; Qfr[3] = -rhow * ww;
+    }
+    break;
+  case BC_SYMP: /* bcflux.f90:272-299 */
+    rhow = Qfl[0]; uw = Qfl[1] / rhow; vw = Qfl[2] / rhow; ww = Qfl[3] / rhow;
+    Qfr[0] = p->rinf * 2.0 - Qfl[0];
+    magnorm = sqrt(normf[0] * normf[0] + normf[1] * normf[1]);
+    VN = uw * normf[0] / magnorm + vw * normf[1] / magnorm;
+    Qfr[1] = Qfr[0] * (uw - 2.0 * VN * normf[0] / magnorm);
+    Qfr[2] = Qfr[0] * (vw - 2.0 * VN * normf[1] / magnorm);
+    Qfr[3] = -Qfr[0] * ww;
+    Qfr[5] = p->bxinf; Qfr[6] = p->byinf; Qfr[7] = -Qfl[7]; Qfr[8] = Qfl[8];
+    Qfr[4] = (p->pinf / (GAM - 1.0) + 0.5 / Qfr[0] * ((Qfr[1] * Qfr[1] + Qfr[2] * Qfr[2])
+                                                      + Qfr[3] * Qfr[3]))
+             + 0.5 * (((Qfr[5] * Qfr[5] + Qfr[6] * Qfr[6]) + Qfr[7] * Qfr[7]) + Qfr[8] * Qfr[8]);
+    break;
+  default: /* FREE, bcflux.f90:355 */
+    for (int v = 0; v < 9; v++) Qfr[v] = Qfl[v];
+  }
+}
+
+static void bcflux_list(orc_ctx *o, const int *list, int n, int type, double *eigvmax,
+                        double *umax) {
+  const int N = o->N;
+  double eig = *eigvmax, um = *umax;
+#pragma omp parallel for reduction(max : eig, um) schedule(static)
+  for (int f = 0; f < n; f++) {
+    int iface = list[f] - 1;
+    int icleft = IF2C(iface, 0) - 1, ifacelc = IF2C(iface, 2);
+    int faml = ifacelc % 2 + 1;
+    for (int nfp = 1; nfp <= N; nfp++) {
+      double Qfl[9], Qfr[9], Fnl[9], Fnr[9], normf[2];
+      int ifpl = o->iface2fp[(nfp - 1) + N * (ifacelc - 1)];
+      int jfpl = o->jface2fp[(nfp - 1) + N * (ifacelc - 1)];
+      int sign_l = 1;
+      trace_q(o, o->Q, icleft, faml, ifpl, jfpl, Qfl);
+      if (faml == 1) { if (ifpl == 1) sign_l = -1; }
+      else { if (jfpl == 1) sign_l = -1; }
+      normal_of(o, faml, ifpl, jfpl, icleft, normf);
+      bc_ghost(o, type, Qfl, normf, Qfr);
+      if (o->p.vismode == 1) {
+        double *a = fluxslot(o, o->Qvfi, o->Qvfj, faml, ifpl, jfpl, icleft);
+        for (int v = 0; v < 9; v++) a[v] = 0.5 * Qfl[v] + 0.5 * Qfr[v];
+      }
+      getrusanovflux(Qfl, Qfr, Fnl, Fnr, normf, sign_l, 1, o->c_h, &eig, &um);
+      double *fl = fluxslot(o, o->F1, o->G2, faml, ifpl, jfpl, icleft);
+      for (int v = 0; v < 9; v++) fl[v] = Fnl[v];
+    }
+  }
+  *eigvmax = eig; *umax = um;
+}
+
+/* BCFLUX (bcflux.f90:1-377), list order INLE, OUTL, WALL, SYMP, FREE */
+static void bcflux(orc_ctx *o, double *eig, double *um) {
+  bcflux_list(o, o->ibfinl, o->ninlet, BC_INLE, eig, um);
+  bcflux_list(o, o->ibfout, o->noutlet, BC_OUTL, eig, um);
+  bcflux_list(o, o->ibfwal, o->nwall, BC_WALL, eig, um);
+  bcflux_list(o, o->ibfsymp, o->nsymp, BC_SYMP, eig, um);
+  bcflux_list(o, o->ibfree, o->nfree, BC_FREE, eig, um);
+}
+
+/* ---- gradients ----------------------------------------------------------- */
+
+/* nablaQsp(1,9) (compflux.f90:107-157) */
+static void nablaQsp(orc_ctx *o) {
+  const int N = o->N;
+#pragma omp parallel for schedule(static)
+  for (int ic = 0; ic < o->nc; ic++)
+    for (int js = 0; js < N; js++)
+      for (int is = 0; is < N; is++)
+        for (int d = 0; d < 2; d++) {
+          double dps[9], det[9];
+          for (int v = 0; v < 9; v++) dps[v] = 0.0;
+          for (int rfp = 0; rfp <= N; rfp++) {
+            double s = o->s1[IS1(0, d, rfp, js, ic)], mm = MM(rfp, is);
+            for (int v = 0; v < 9; v++) dps[v] = dps[v] + o->Qvfi[IF1(v, rfp, js, ic)] * mm * s;
+          }
+          for (int v = 0; v < 9; v++) det[v] = 0.0;
+          for (int rfp = 0; rfp <= N; rfp++) {
+            double s = o->s2[IS2(1, d, is, rfp, ic)], mm = MM(rfp, js);
+            for (int v = 0; v < 9; v++) det[v] = det[v] + o->Qvfj[IG2(v, is, rfp, ic)] * mm * s;
+          }
+          double J = o->jac[IJ(is, js, ic)];
+          for (int v = 0; v < 9; v++) o->nablaQs[INQ(v, d, is, js, ic)] = (dps[v] + det[v]) / J;
+        }
+}
+
+/* internal energy of a flux-point state, as written in
+ * compflux.f90:275-280 */
+static double enei_of(const double *q) {
+  return (((q[4] / q[0] - 0.5 * ((q[1] / q[0]) * (q[1] / q[0]))) -
+           0.5 * ((q[2] / q[0]) * (q[2] / q[0]))) -
+          0.5 * ((q[3] / q[0]) * (q[3] / q[0]))) -
+         0.5 * (((q[5] * q[5] + q[6] * q[6]) + q[7] * q[7]) + q[8] * q[8]) / q[0];
+}
+
+/* comp_grad_internal_energy (compflux.f90:260-299) */
+static void comp_grad_internal_energy(orc_ctx *o) {
+  const int N = o->N;
+#pragma omp parallel for schedule(static)
+  for (int ic = 0; ic < o->nc; ic++)
+    for (int js = 0; js < N; js++)
+      for (int is = 0; is < N; is++) {
+        double t = 0.0;
+        for (int rfp = 0; rfp <= N; rfp++)
+          t = t + enei_of(&o->Qvfi[IF1(0, rfp, js, ic)]) * MM(rfp, is);
+        o->dedxi[IJ(is, js, ic)] = t;
+        t = 0.0;
+        for (int rfp = 0; rfp <= N; rfp++)
+          t = t + enei_of(&o->Qvfj[IG2(0, is, rfp, ic)]) * MM(rfp, js);
+        o->dedeta[IJ(is, js, ic)] = t;
+      }
+}
+
+/* nablaQ_interior_FP (compflux.f90:159-196) */
+static void nablaQ_interior_FP(orc_ctx *o) {
+  const int N = o->N;
+#pragma omp parallel for schedule(static)
+  for (int ic = 0; ic < o->nc; ic++) {
+    double a[9], b[9];
+    for (int jfp = 0; jfp < N; jfp++)
+      for (int ifp = 1; ifp < N; ifp++) {
+        trace_grad(o, ic, 1, ifp + 1, jfp + 1, a, b);
+        for (int v = 0; v < 9; v++) {
+          o->nablaQvfi[INQI(v, 0, ifp, jfp, ic)] = a[v];
+          o->nablaQvfi[INQI(v, 1, ifp, jfp, ic)] = b[v];
+        }
+      }
+    for (int jfp = 1; jfp < N; jfp++)
+      for (int ifp = 0; ifp < N; ifp++) {
+        trace_grad(o, ic, 2, ifp + 1, jfp + 1, a, b);
+        for (int v = 0; v < 9; v++) {
+          o->nablaQvfj[INQJ(v, 0, ifp, jfp, ic)] = a[v];
+          o->nablaQvfj[INQJ(v, 1, ifp, jfp, ic)] = b[v];
+        }
+      }
+  }
+}
+
+static double *gradslot(orc_ctx *o, int fam, int ifp, int jfp, int ic, int d) {
+  return (fam == 1) ? &o->nablaQvfi[INQI(0, d, ifp - 1, jfp - 1, ic)]
+                    : &o->nablaQvfj[INQJ(0, d, ifp - 1, jfp - 1, ic)];
+}
+
+/* interfaceVISflux / CYCLOCVISFLUX (interfaceflux_all.f90:114-195,
+ * cyclocflux_all.f90:114-204) */
+static void two_cell_visface(orc_ctx *o, int icleft, int icright, int ifacelc, int ifacerc) {
+  const int N = o->N;
+  int faml = ifacelc % 2 + 1, famr = ifacerc % 2 + 1;
+  for (int nfp = 1; nfp <= N; nfp++) {
+    double Qfl[9], Qfl2[9], Qfr[9], Qfr2[9];
+    int ifpl = o->iface2fp[(nfp - 1) + N * (ifacelc - 1)];
+    int jfpl = o->jface2fp[(nfp - 1) + N * (ifacelc - 1)];
+    trace_grad(o, icleft, faml, ifpl, jfpl, Qfl, Qfl2);
+    int ifpr = o->iface2fp[(N - nfp) + N * (ifacerc - 1)];
+    int jfpr = o->jface2fp[(N - nfp) + N * (ifacerc - 1)];
+    trace_grad(o, icright, famr, ifpr, jfpr, Qfr, Qfr2);
+    double *l1 = gradslot(o, faml, ifpl, jfpl, icleft, 0), *l2 = gradslot(o, faml, ifpl, jfpl, icleft, 1);
+    double *r1 = gradslot(o, famr, ifpr, jfpr, icright, 0), *r2 = gradslot(o, famr, ifpr, jfpr, icright, 1);
+    for (int v = 0; v < 9; v++) {
+      double a = 0.5 * Qfl[v] + 0.5 * Qfr[v], b = 0.5 * Qfl2[v] + 0.5 * Qfr2[v];
+      l1[v] = a; l2[v] = b; r1[v] = a; r2[v] = b;
+    }
+  }
+}
+
+/* BCVISFLUX (bcflux.f90:380-666) */
+static void bcvis_list(orc_ctx *o, const int *list, int n, int wall) {
+  const int N = o->N;
+#pragma omp parallel for schedule(static)
+  for (int f = 0; f < n; f++) {
+    int iface = list[f] - 1;
+    int ic = IF2C(iface, 0) - 1, ifacelc = IF2C(iface, 2);
+    int faml = ifacelc % 2 + 1;
+    for (int nfp = 1; nfp <= N; nfp++) {
+      double Qfl[9], Qfl2[9], Qfr[9], Qfr2[9], Qfo[9];
+      int ifpl = o->iface2fp[(nfp - 1) + N * (ifacelc - 1)];
+      int jfpl = o->jface2fp[(nfp - 1) + N * (ifacelc - 1)];
+      trace_grad(o, ic, faml, ifpl, jfpl, Qfl, Qfl2);
+      if (!wall) {
+        for (int v = 0; v < 9; v++) { Qfr[v] = Qfl[v]; Qfr2[v] = Qfl2[v]; }
+      } else {
+        /* adiabatic wall, bcflux.f90:484-571 */
+        double dxf = 0.0, def = 0.0;
+        for (int v = 0; v < 9; v++) Qfo[v] = 0.0;
+        for (int sp = 0; sp < N; sp++) {
+          const double *q;
+          double l, dx, de;
+          if (faml == 1) {
+            q = &o->Q[IQ(0, sp, jfpl - 1, ic)];
+            l = LM(ifpl - 1, sp);
+            dx = o->dedxi[IJ(sp, jfpl - 1, ic)]; de = o->dedeta[IJ(sp, jfpl - 1, ic)];
+          } else {
+            q = &o->Q[IQ(0, ifpl - 1, sp, ic)];
+            l = LM(jfpl - 1, sp);
+            dx = o->dedxi[IJ(ifpl - 1, sp, ic)]; de = o->dedeta[IJ(ifpl - 1, sp, ic)];
+          }
+          for (int v = 0; v < 9; v++) Qfo[v] = Qfo[v] + q[v] * l;
+          dxf = dxf + dx * l;
+          def = def + de * l;
+        }
+        double ur = Qfo[1] / Qfo[0], vr = Qfo[2] / Qfo[0], wr = Qfo[3] / Qfo[0];
+        double inte = (Qfo[4] / Qfo[0] - 0.5 * ((ur * ur + vr * vr) + wr * wr)) -
+                      0.5 * (((Qfo[5] * Qfo[5] + Qfo[6] * Qfo[6]) + Qfo[7] * Qfo[7]) + Qfo[8] * Qfo[8]) / Qfo[0];
+        for (int v = 0; v < 9; v++) if (v != 4) { Qfr[v] = Qfl[v]; Qfr2[v] = Qfl2[v]; }
+        double dur = (Qfl[1] - ur * Qfl[0]) / Qfo[0];
+        double dvr = (Qfl[2] - vr * Qfl[0]) / Qfo[0];
+        double dwr = (Qfl[3] - wr * Qfl[0]) / Qfo[0];
+        double dKE = 0.5 * ((ur * ur + vr * vr) + wr * wr) * Qfr[0] +
+                     Qfo[0] * ((ur * dur + vr * dvr) + wr * dwr);
+        double dBp = ((Qfo[5] * Qfl[5] + Qfo[6] * Qfl[6]) + Qfo[7] * Qfl[7]) + Qfo[8] * Qfl[8];
+        if (faml == 1)
+          Qfr[4] = (((-Qfl[4] + 2.0 * inte * Qfr[0]) + 2.0 * dKE) + dBp) +
+                   2.0 * def * o->s1[IS1(1, 0, ifpl - 1, jfpl - 1, ic)] * Qfo[0];
+        else
+          Qfr[4] = (((-Qfl[4] + 2.0 * inte * Qfr[0]) + 2.0 * dKE) + dBp) +
+                   2.0 * dxf * o->s2[IS2(0, 0, ifpl - 1, jfpl - 1, ic)] * Qfo[0];
+        dur = (Qfl2[1] - ur * Qfl2[0]) / Qfo[0];
+        dvr = (Qfl2[2] - vr * Qfl2[0]) / Qfo[0];
+        dwr = (Qfl2[3] - wr * Qfl2[0]) / Qfo[0];
+        dKE = 0.5 * ((ur * ur + vr * vr) + wr * wr) * Qfr2[0] + Qfo[0] * ((ur * dur + vr * dvr) + wr * dwr);
+        /* the reference reuses the first-direction dBp here (bcflux.f90:563) */
+        dBp = ((Qfo[5] * Qfl[5] + Qfo[6] * Qfl[6]) + Qfo[7] * Qfl[7]) + Qfo[8] * Qfl[8];
+        if (faml == 1)
+          Qfr2[4] = (((-Qfl2[4] + 2.0 * inte * Qfr2[0]) + 2.0 * dKE) + dBp) +
+                    2.0 * def * o->s1[IS1(1, 1, ifpl - 1, jfpl - 1, ic)] * Qfo[0];
+        else
+          Qfr2[4] = (((-Qfl2[4] + 2.0 * inte * Qfr2[0]) + 2.0 * dKE) + dBp) +
+                    2.0 * dxf * o->s2[IS2(0, 1, ifpl - 1, jfpl - 1, ic)] * Qfo[0];
+      }
+      double *l1 = gradslot(o, faml, ifpl, jfpl, ic, 0), *l2 = gradslot(o, faml, ifpl, jfpl, ic, 1);
+      for (int v = 0; v < 9; v++) {
+        l1[v] = 0.5 * Qfl[v] + 0.5 * Qfr[v];
+        l2[v] = 0.5 * Qfl2[v] + 0.5 * Qfr2[v];
+      }
+    }
+  }
+}
+
+/* ---- artificial viscosity sensor (AV.f90) ------------------------------- */
+
+/* comp_artificial_viscosity (AV.f90:1-142); the MPI reductions of Se_min /
+ * Se_max (AV.f90:99-104) have no effect on any output and are omitted. */
+static void comp_artificial_viscosity(orc_ctx *o) {
+  const int N = o->N, Nl = N - 1;
+#pragma omp parallel for schedule(static)
+  for (int ic = 0; ic < o->nc; ic++) {
+    double low[2][8][8], high[2][8][8];
+    for (int jsl = 0; jsl < Nl; jsl++)
+      for (int isl = 0; isl < Nl; isl++) {
+        double q1 = 0.0, q2 = 0.0;
+        for (int js = 0; js < N; js++)
+          for (int is = 0; is < N; is++) {
+            double hx = o->hlow[is + N * isl], hy = o->hlow[js + N * jsl];
+            q1 = q1 + o->Q[IQ(0, is, js, ic)] * hx * hy;
+            double jz = o->nablaQs[INQ(6, 0, is, js, ic)] - o->nablaQs[INQ(5, 1, is, js, ic)];
+            q2 = q2 + (jz * jz) * hx * hy;
+          }
+        low[0][isl][jsl] = q1; low[1][isl][jsl] = q2;
+      }
+    for (int js = 0; js < N; js++)
+      for (int is = 0; is < N; is++) {
+        double a = 0.0, b = 0.0;
+        for (int jsl = 0; jsl < Nl; jsl++)
+          for (int isl = 0; isl < Nl; isl++) {
+            double hx = o->hhigh[isl + Nl * is], hy = o->hhigh[jsl + Nl * js];
+            a = a + low[0][isl][jsl] * hx * hy;
+            b = b + low[1][isl][jsl] * hx * hy;
+          }
+        high[0][is][js] = a; high[1][is][js] = b;
+      }
+    double Se1 = 0.0, Se2 = 0.0;
+    for (int js = 0; js < N; js++)
+      for (int is = 0; is < N; is++) {
+        double q = o->Q[IQ(0, is, js, ic)];
+        double d = high[0][is][js] - q;
+        Se1 = Se1 + (d * d) / (q * q);
+        double jz = o->nablaQs[INQ(6, 0, is, js, ic)] - o->nablaQs[INQ(5, 1, is, js, ic)];
+        double Jz = jz * jz;
+        double e = high[1][is][js] - Jz;
+        Se2 = Se2 + (e * e) / (Jz * Jz);
+      }
+    o->SM_IND[0 + 2 * (size_t)ic] = log10(Se1);
+    o->SM_IND[1 + 2 * (size_t)ic] = log10(Se2);
+  }
+  /* AV.f90:106-127 */
+  const double se_low[2] = {-5.0, -3.0}, se_high[2] = {0.0, 1.0};
+  double k_range[2], s0[2];
+  for (int v = 0; v < 2; v++) {
+    k_range[v] = (se_high[v] - se_low[v]) * 0.5;
+    s0[v] = (se_high[v] + se_low[v]) * 0.5;
+  }
+  double h_msh = sqrt(o->p.dxcycl * o->p.dycycl / (double)o->p.ncellglob);
+  o->epsilon0 = o->eigvmax_g * h_msh / (double)(N - 1) * 0.05;
+  for (int ic = 0; ic < o->nc; ic++)
+    for (int v = 0; v < 2; v++) {
+      double S = o->SM_IND[v + 2 * (size_t)ic], mu;
+      if (S < se_low[v]) mu = 0.0;
+      else if (S > se_high[v]) mu = o->epsilon0;
+      else mu = o->epsilon0 / 2.0 * (1.0 + sin(PI_ * (S - s0[v]) / 2.0 / k_range[v]));
+      o->muAV[v + 2 * (size_t)ic] = mu;
+    }
+}
+
+/* muAV of local face k (1..4) of cell ic as selected in AV.f90:213-222 */
+static double *muav_face(orc_ctx *o, int ic, int k) {
+  switch (k) {
+  case 1: return &o->muAVfj[MUFI(0, 0, ic)];
+  case 2: return &o->muAVfi[MUFI(0, 1, ic)];
+  case 3: return &o->muAVfj[MUFI(0, 1, ic)];
+  default: return &o->muAVfi[MUFI(0, 0, ic)];
+  }
+}
+
+/* compAVflux + interfaceAVflux + cyclocAVflux (AV.f90:167-271,462-532) */
+static void compAVflux(orc_ctx *o) {
+  for (int ic = 0; ic < o->nc; ic++)
+    for (int f = 0; f < 2; f++)
+      for (int v = 0; v < 2; v++) {
+        o->muAVfi[MUFI(v, f, ic)] = o->muAV[v + 2 * (size_t)ic];
+        o->muAVfj[MUFI(v, f, ic)] = o->muAV[v + 2 * (size_t)ic];
+      }
+  for (int pass = 0; pass < 2; pass++) {
+    int n = pass == 0 ? o->ninter : o->npair;
+    for (int f = 0; f < n; f++) {
+      int icl, icr, kl, kr;
+      if (pass == 0) {
+        int iface = o->ibfinter[f] - 1;
+        icl = IF2C(iface, 0) - 1; icr = IF2C(iface, 1) - 1; kl = IF2C(iface, 2); kr = IF2C(iface, 3);
+      } else {
+        int K1 = o->cyc2fpair[f] - 1, K2 = o->cyc2fpair[f + o->npair] - 1;
+        icl = IF2C(K1, 0) - 1; icr = IF2C(K2, 0) - 1; kl = IF2C(K1, 2); kr = IF2C(K2, 2);
+      }
+      double *a = muav_face(o, icl, kl), *b = muav_face(o, icr, kr);
+      double l0 = a[0], l1 = a[1], r0 = b[0], r1 = b[1];
+      a[0] = 0.5 * (l0 + r0); a[1] = 0.5 * (l1 + r1);
+      b[0] = 0.5 * (l0 + r0); b[1] = 0.5 * (l1 + r1);
+    }
+  }
+}
+
+/* comp_viscous_flux (compflux.f90:198-258); the AV flux term uses
+ * mu_art = eta_art = 0 (getrusanov.f90:325-328) and so adds
+ * getVflux(...,0,0) which is zero unless a gradient is not finite. */
+static void comp_viscous_flux(orc_ctx *o) {
+  const int N = o->N;
+  const double mu = o->p.mu, eta0 = o->p.eta0;
+  const int av = o->p.artiv == 1;
+#pragma omp parallel for schedule(static)
+  for (int ic = 0; ic < o->nc; ic++) {
+    double nq[18], Fv[9], Gv[9], Fa[9], Ga[9];
+    for (int jfp = 0; jfp < N; jfp++)
+      for (int ifp = 0; ifp <= N; ifp++) {
+        for (int v = 0; v < 9; v++) {
+          nq[v] = o->nablaQvfi[INQI(v, 0, ifp, jfp, ic)];
+          nq[9 + v] = o->nablaQvfi[INQI(v, 1, ifp, jfp, ic)];
+        }
+        const double *qs = &o->Qvfi[IF1(0, ifp, jfp, ic)];
+        getVflux(qs, nq, mu, eta0, Fv, Gv);
+        double a = o->s1[IS1(0, 0, ifp, jfp, ic)], b = o->s1[IS1(0, 1, ifp, jfp, ic)];
+        double *out = &o->Fv1[IF1(0, ifp, jfp, ic)];
+        for (int v = 0; v < 9; v++) out[v] = Fv[v] * a + Gv[v] * b;
+        if (av) {
+          getVflux(qs, nq, 0.0, 0.0, Fa, Ga);
+          for (int v = 0; v < 9; v++) out[v] = (out[v] + Fa[v] * a) + Ga[v] * b;
+        }
+      }
+    for (int jfp = 0; jfp <= N; jfp++)
+      for (int ifp = 0; ifp < N; ifp++) {
+        for (int v = 0; v < 9; v++) {
+          nq[v] = o->nablaQvfj[INQJ(v, 0, ifp, jfp, ic)];
+          nq[9 + v] = o->nablaQvfj[INQJ(v, 1, ifp, jfp, ic)];
+        }
+        const double *qs = &o->Qvfj[IG2(0, ifp, jfp, ic)];
+        getVflux(qs, nq, mu, eta0, Fv, Gv);
+        double a = o->s2[IS2(1, 0, ifp, jfp, ic)], b = o->s2[IS2(1, 1, ifp, jfp, ic)];
+        double *out = &o->Gv2[IG2(0, ifp, jfp, ic)];
+        for (int v = 0; v < 9; v++) out[v] = Fv[v] * a + Gv[v] * b;
+        if (av) {
+          getVflux(qs, nq, 0.0, 0.0, Fa, Ga);
+          for (int v = 0; v < 9; v++) out[v] = (out[v] + Fa[v] * a) + Ga[v] * b;
+        }
+      }
+  }
+}
+
+/* ---- compflux / compresid ------------------------------------------------ */
+
+/* compflux (compflux.f90:1-33) */
+static void compflux(orc_ctx *o) {
+  o->c_h = sqrt(o->eigvmax_g * (o->eigvmax_g - o->umax_g));
+  double eig = o->eigvmax, um = o->umax;
+  inviscid_flux_interior(o, &um);
+  interfaceflux(o, &eig, &um);
+  cyclocflux(o, &eig, &um);
+  bcflux(o, &eig, &um);
+  o->eigvmax = eig; o->umax = um;
+  o->eigvmax_g = o->eigvmax;   /* MPI_ALLREDUCE(MAX) on one rank */
+  o->umax_g = o->umax;
+  if (o->p.vismode == 1) {
+    nablaQsp(o);
+    if (o->nwall > 0) comp_grad_internal_energy(o);
+    nablaQ_interior_FP(o);
+#pragma omp parallel for schedule(static)
+    for (int f = 0; f < o->ninter; f++) {
+      int iface = o->ibfinter[f] - 1;
+      two_cell_visface(o, IF2C(iface, 0) - 1, IF2C(iface, 1) - 1, IF2C(iface, 2), IF2C(iface, 3));
+    }
+    bcvis_list(o, o->ibfinl, o->ninlet, 0);
+    bcvis_list(o, o->ibfout, o->noutlet, 0);
+    bcvis_list(o, o->ibfwal, o->nwall, 1);
+    bcvis_list(o, o->ibfsymp, o->nsymp, 0);
+    bcvis_list(o, o->ibfree, o->nfree, 0);
+#pragma omp parallel for schedule(static)
+    for (int ip = 0; ip < o->npair; ip++) {
+      int K1 = o->cyc2fpair[ip] - 1, K2 = o->cyc2fpair[ip + o->npair] - 1;
+      two_cell_visface(o, IF2C(K1, 0) - 1, IF2C(K2, 0) - 1, IF2C(K1, 2), IF2C(K2, 2));
+    }
+    if (o->p.artiv == 1) {
+      comp_artificial_viscosity(o);
+      compAVflux(o);
+    }
+    comp_viscous_flux(o);
+  }
+}
+
+/* compresid (compresid.f90:1-80) */
+static void compresid(orc_ctx *o) {
+  const int N = o->N;
+  compflux(o);
+  const int vis = o->p.vismode == 1;
+  const double alpha = o->p.alpha;
+#pragma omp parallel for schedule(static)
+  for (int ic = 0; ic < o->nc; ic++)
+    for (int js = 0; js < N; js++)
+      for (int is = 0; is < N; is++) {
+        double dF[9], dG[9], dFv[9], dGv[9];
+        for (int v = 0; v < 9; v++) { dF[v] = 0.0; dFv[v] = 0.0; }
+        for (int rfp = 0; rfp <= N; rfp++)
+          for (int v = 0; v < 9; v++) {
+            dF[v] = dF[v] + o->F1[IF1(v, rfp, js, ic)] * MM(rfp, is);
+            if (vis) dFv[v] = dFv[v] + o->Fv1[IF1(v, rfp, js, ic)] * MM(rfp, is);
+          }
+        for (int v = 0; v < 9; v++) { dG[v] = 0.0; dGv[v] = 0.0; }
+        for (int rfp = 0; rfp <= N; rfp++)
+          for (int v = 0; v < 9; v++) {
+            dG[v] = dG[v] + o->G2[IG2(v, is, rfp, ic)] * MM(rfp, js);
+            if (vis) dGv[v] = dGv[v] + o->Gv2[IG2(v, is, rfp, ic)] * MM(rfp, js);
+          }
+        double *r = &o->resid[IQ(0, is, js, ic)];
+        for (int v = 0; v < 9; v++) r[v] = -(dF[v] + dG[v]);
+        if (vis)
+          for (int v = 0; v < 9; v++) r[v] = r[v] + (dFv[v] + dGv[v]);
+      }
+#pragma omp parallel for schedule(static)
+  for (int ic = 0; ic < o->nc; ic++)
+    for (int js = 0; js < N; js++)
+      for (int is = 0; is < N; is++) {
+        double *r = &o->resid[IQ(0, is, js, ic)];
+        const double *q = &o->Q[IQ(0, is, js, ic)];
+        double J = o->jac[IJ(is, js, ic)];
+        for (int v = 0; v < 9; v++) r[v] = r[v] / J;
+        double divB = o->nablaQs[INQ(5, 0, is, js, ic)] + o->nablaQs[INQ(6, 1, is, js, ic)];
+        double Bx = q[5], By = q[6], Bz = q[7];
+        double ux = q[1] / q[0], uy = q[2] / q[0], uz = q[3] / q[0], psi = q[8];
+        r[1] = r[1] - divB * Bx;
+        r[2] = r[2] - divB * By;
+        r[3] = r[3] - divB * Bz;
+        r[4] = r[4] - divB * ((ux * Bx + uy * By) + uz * Bz);
+        r[5] = r[5] - divB * ux;
+        r[6] = r[6] - divB * uy;
+        r[7] = r[7] - divB * uz;
+        r[8] = r[8] - alpha * psi;
+      }
+}
+
+int orc_compresid(orc_ctx *o, double *resid, double *nablaqs) {
+  compresid(o);
+  size_t nq = (size_t)9 * o->N * o->N * o->nc;
+  if (resid) memcpy(resid, o->resid, sizeof(double) * nq);
+  if (nablaqs) memcpy(nablaqs, o->nablaQs, sizeof(double) * 2 * nq);
+  return 0;
+}
+
+/* ---- time integration (iterations.f90) ----------------------------------- */
+
+static void axpy_update(orc_ctx *o, double *Q, const double *A, double a, const double *B,
+                        double b, double c, const double *R) {
+  /* Q = a*A + b*B + c*R, evaluated as ((a*A) + (b*B)) + c*R; A or B may be
+   * NULL meaning that term is absent (then it is Q + c*R when both are
+   * NULL). */
+  size_t n = (size_t)9 * o->N * o->N * o->nc;
+#pragma omp parallel for schedule(static)
+  for (size_t i = 0; i < n; i++) {
+    if (!A && !B) Q[i] = Q[i] + c * R[i];
+    else if (!B) Q[i] = a * A[i] + c * R[i];
+    else Q[i] = (a * A[i] + b * B[i]) + c * R[i];
+  }
+}
+
+/* RK_S5P3 (iterations.f90:158-213) */
+static void rk_s5p3(orc_ctx *o, double *Qk0, double *Qka) {
+  const double a30 = 0.355909775063327, a32 = 0.644090224936674;
+  const double a40 = 0.367933791638137, a43 = 0.632066208361863;
+  const double a52 = 0.237593836598569, a54 = 0.762406163401431;
+  const double b10 = 0.377268915331368, b21 = 0.377268915331368;
+  const double b32 = 0.242995220537396, b43 = 0.238458932846290;
+  const double b54 = 0.287632146308408;
+  size_t n = (size_t)9 * o->N * o->N * o->nc;
+  double dt = o->p.dt, ctime0 = o->ctime, dt1, dt2, dt3, dt4;
+  memcpy(Qk0, o->Q, sizeof(double) * n);
+  compresid(o);
+  axpy_update(o, o->Q, NULL, 0, NULL, 0, b10 * dt, o->resid);
+  dt1 = b10 * dt; o->ctime = ctime0 + dt1;
+  compresid(o);
+  axpy_update(o, o->Q, NULL, 0, NULL, 0, b21 * dt, o->resid);
+  dt2 = dt1 + b21 * dt; o->ctime = ctime0 + dt2;
+  memcpy(Qka, o->Q, sizeof(double) * n);
+  compresid(o);
+  axpy_update(o, o->Q, Qk0, a30, o->Q, a32, b32 * dt, o->resid);
+  dt3 = a32 * dt2 + b32 * dt; o->ctime = ctime0 + dt3;
+  compresid(o);
+  axpy_update(o, o->Q, Qk0, a40, o->Q, a43, b43 * dt, o->resid);
+  dt4 = a43 * dt3 + b43 * dt; o->ctime = ctime0 + dt4;
+  compresid(o);
+  axpy_update(o, o->Q, Qka, a52, o->Q, a54, b54 * dt, o->resid);
+  o->ctime = ctime0 + dt;
+}
+
+/* FiveStageRk (iterations.f90:88-156) */
+static void five_stage_rk(orc_ctx *o, double *qj0, double *qj2, double *qj3, double *resid0) {
+  size_t n = (size_t)9 * o->N * o->N * o->nc;
+  double dt = o->p.dt, old_time = o->ctime, rdt;
+  double *Q = o->Q, *R = o->resid;
+  memcpy(qj0, Q, sizeof(double) * n);
+  compresid(o);
+  const double a11 = 0.39175222700392;
+  rdt = dt * a11;
+#pragma omp parallel for schedule(static)
+  for (size_t i = 0; i < n; i++) Q[i] = qj0[i] + rdt * R[i];
+  o->ctime = old_time + rdt;
+  compresid(o);
+  const double a21 = 0.44437049406734, a22 = 0.55562950593266, a23 = 0.36841059262959;
+#pragma omp parallel for schedule(static)
+  for (size_t i = 0; i < n; i++) Q[i] = (a21 * qj0[i] + a22 * Q[i]) + a23 * R[i] * dt;
+  memcpy(qj2, Q, sizeof(double) * n);
+  rdt = rdt * a22 + a23 * dt; o->ctime = old_time + rdt;
+  compresid(o);
+  const double a31 = 0.6201018513854, a32 = 0.3798981486146, a33 = 0.25189177424738;
+#pragma omp parallel for schedule(static)
+  for (size_t i = 0; i < n; i++) Q[i] = (a31 * qj0[i] + a32 * Q[i]) + a33 * dt * R[i];
+  memcpy(qj3, Q, sizeof(double) * n);
+  rdt = rdt * a32 + a33 * dt; o->ctime = old_time + rdt;
+  compresid(o);
+  const double a41 = 0.17807995410773, a42 = 0.82192004589227, a43 = 0.54497475021237;
+#pragma omp parallel for schedule(static)
+  for (size_t i = 0; i < n; i++) Q[i] = (a41 * qj0[i] + a42 * Q[i]) + a43 * dt * R[i];
+  memcpy(resid0, R, sizeof(double) * n);
+  rdt = rdt * a42 + a43 * dt; o->ctime = old_time + rdt;
+  compresid(o);
+  const double a51 = 0.00683325884039, a52 = 0.51723167208978, a53 = 0.12759831133288;
+  const double a54 = 0.34833675773694, a55 = 0.08460416338212, a56 = 0.22600748319395;
+#pragma omp parallel for schedule(static)
+  for (size_t i = 0; i < n; i++)
+    Q[i] = ((((a51 * qj0[i] + a52 * qj2[i]) + a53 * qj3[i]) + a54 * Q[i]) + a55 * dt * resid0[i]) +
+           a56 * dt * R[i];
+  o->ctime = old_time + dt;
+}
+
+/* residual / divB norms of iterations.f90:34-62, serial sum in the
+ * reference's order */
+static void norms(orc_ctx *o) {
+  const int N = o->N;
+  double rn = 0.0, db = 0.0;
+  for (int ic = 0; ic < o->nc; ic++)
+    for (int js = 0; js < N; js++)
+      for (int is = 0; is < N; is++) {
+        for (int k = 0; k < 9; k++) {
+          double r = o->resid[IQ(k, is, js, ic)];
+          rn = rn + r * r;
+        }
+        double d = o->nablaQs[INQ(5, 0, is, js, ic)] + o->nablaQs[INQ(6, 1, is, js, ic)];
+        db = db + d * d;
+      }
+  double resglob = 0.0 + rn, divglob = 0.0 + db;
+  o->resid_rms = sqrt(resglob / (double)(o->p.ncellglob * N * N * 9));
+  o->divb_rms = sqrt(divglob / (double)(o->p.ncellglob * N * N));
+}
+
+int orc_step(orc_ctx *o, int nsteps, int norm_every) {
+  size_t n = (size_t)9 * o->N * o->N * o->nc;
+  double *w0 = zd(n), *w1 = zd(n), *w2 = NULL, *w3 = NULL;
+  if (o->p.k_stage == 5) { w2 = zd(n); w3 = zd(n); }
+  for (int s = 0; s < nsteps; s++) {
+    o->iter = o->iter + 1;
+    if (o->p.k_stage == 5) five_stage_rk(o, w0, w1, w2, w3);
+    else rk_s5p3(o, w0, w1);
+    if (norm_every > 0 && o->iter % norm_every == 1) norms(o);
+  }
+  free(w0); free(w1); free(w2); free(w3);
+  return 0;
+}
+
+void orc_diag(orc_ctx *o, double *c_h, double *epsilon0, double *resid_rms, double *divb_rms) {
+  if (c_h) *c_h = o->c_h;
+  if (epsilon0) *epsilon0 = o->epsilon0;
+  if (resid_rms) *resid_rms = o->resid_rms;
+  if (divb_rms) *divb_rms = o->divb_rms;
+}
+
+int orc_get_av(orc_ctx *o, double *sm_ind, double *muav) {
+  if (sm_ind) memcpy(sm_ind, o->SM_IND, sizeof(double) * 2 * (size_t)o->nc);
+  if (muav) memcpy(muav, o->muAV, sizeof(double) * 2 * (size_t)o->nc);
+  return 0;
+}
