@@ -1,0 +1,495 @@
+// sd2d.hip - C ABI (include/sd2d.h) of the MI355X SD2D-MHD residual and
+// RK path: context set-up, device layouts, and the launch sequence.
+//
+// Device layout ("tile-major SoA"): cells are grouped in tiles of CPW
+// cells (the cells of one wave); inside a tile every (variable, point)
+// pair is a contiguous run of CPW doubles.  A wave therefore reads and
+// writes whole runs, and a tile of state is one contiguous block.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/sd2d.h"
+#include "sd2d_host.hpp"
+#include "sd2d_stage.hpp"
+#include "sd2d_rkseq.hpp"
+
+using namespace sd2d;
+
+struct sd2d_ctx {
+  int N = 0, ncell = 0, ntile = 0, cpw = 0;
+  sd2d_params p{};
+  int device = 0;
+  hipStream_t stream = nullptr;
+  Ops ops{};
+  Phys ph{};
+  double *q[3] = {nullptr, nullptr, nullptr};  // RK state buffers
+  int cur = 0;                                  // buffer holding the state
+  double *fsr[4] = {nullptr, nullptr, nullptr, nullptr};  // FiveStageRk registers
+  double *met = nullptr;
+  int *ftab = nullptr;
+  double *avtab = nullptr;
+  double *gmax = nullptr;                       // eigvmax_g, umax_g
+  unsigned long long *slots = nullptr;
+  double *sm_ind = nullptr;
+  double *normbuf = nullptr, *normout = nullptr;
+  double *stagef = nullptr;                     // Fortran-layout staging (Q or resid)
+  double *nablaf = nullptr;                     // Fortran-layout nablaQs
+  double ctime = 0.0;
+  int iter = 0;
+  double last_c_h = 0.0;
+  double resid_rms = 0.0, divb_rms = 0.0;
+  int norms_valid = 0;
+  bool have_av = false;
+  std::string err;
+};
+
+static std::string g_err = "no error";
+
+#define HIPCHK(ctx, call)                                                       \
+  do {                                                                          \
+    hipError_t e_ = (call);                                                     \
+    if (e_ != hipSuccess) {                                                     \
+      (ctx)->err = std::string(#call) + ": " + hipGetErrorString(e_);           \
+      return SD2D_EHIP;                                                         \
+    }                                                                           \
+  } while (0)
+
+static int fail(sd2d_ctx *c, int code, const std::string &m) {
+  if (c) c->err = m; else g_err = m;
+  return code;
+}
+
+// ---------------------------------------------------------------------------
+// small device kernels
+// ---------------------------------------------------------------------------
+// Fortran Q(9,N,N,NCELL) -> tile-major SoA (padded cells get free-stream-free
+// zeros; they are never read by a valid lane).
+template <int N>
+__global__ void k_f2t(const double *__restrict__ f, double *__restrict__ t, int ncell,
+                      int ntile) {
+  constexpr int CPW = Geo<N>::CPW, N2 = Geo<N>::N2;
+  const size_t tot = (size_t)ntile * CPW * 9 * N2;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < tot;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const int ci = int(i % CPW);
+    const size_t r = i / CPW;
+    const int vp = int(r % (9 * N2));
+    const size_t tile = r / (9 * N2);
+    const int v = vp / N2, p = vp % N2;
+    const size_t cell = tile * CPW + ci;
+    t[i] = cell < (size_t)ncell ? f[v + 9 * (p + (size_t)N2 * cell)] : 0.0;
+  }
+}
+
+template <int N>
+__global__ void k_t2f(const double *__restrict__ t, double *__restrict__ f, int ncell) {
+  constexpr int CPW = Geo<N>::CPW, N2 = Geo<N>::N2;
+  const size_t tot = (size_t)ncell * 9 * N2;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < tot;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const int v = int(i % 9);
+    const size_t r = i / 9;
+    const int p = int(r % N2);
+    const size_t cell = r / N2;
+    const size_t tile = cell / CPW, ci = cell % CPW;
+    f[i] = t[(tile * 9 * N2 + v * N2 + p) * CPW + ci];
+  }
+}
+
+// running maxima: g = max over slots (single block)
+__global__ void k_finalize_max(const unsigned long long *__restrict__ slots,
+                               double *__restrict__ gmax) {
+  __shared__ unsigned long long s[2][NSLOT];
+  const int t = threadIdx.x;
+  s[0][t] = slots[2 * t];
+  s[1][t] = slots[2 * t + 1];
+  __syncthreads();
+  for (int o = NSLOT / 2; o > 0; o >>= 1) {
+    if (t < o) {
+      s[0][t] = s[0][t] > s[0][t + o] ? s[0][t] : s[0][t + o];
+      s[1][t] = s[1][t] > s[1][t + o] ? s[1][t] : s[1][t + o];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    gmax[0] = __longlong_as_double((long long)s[0][0]);
+    gmax[1] = __longlong_as_double((long long)s[1][0]);
+  }
+}
+
+// set the running maxima (slot 0 = given local maxima, others 0) and g
+__global__ void k_set_max(unsigned long long *slots, double *gmax, double e, double u,
+                          double eg, double ug) {
+  const int t = threadIdx.x;
+  slots[2 * t] = t == 0 ? (unsigned long long)__double_as_longlong(e) : 0ull;
+  slots[2 * t + 1] = t == 0 ? (unsigned long long)__double_as_longlong(u) : 0ull;
+  if (t == 0) { gmax[0] = eg; gmax[1] = ug; }
+}
+
+// deterministic sum of the per-tile norm partials (single block, fixed order)
+__global__ void k_norm_reduce(const double *__restrict__ nb, int ntile,
+                              double *__restrict__ out) {
+  __shared__ double s[2][256];
+  const int t = threadIdx.x;
+  double a = 0.0, b = 0.0;
+  for (int i = t; i < ntile; i += 256) { a += nb[2 * i]; b += nb[2 * i + 1]; }
+  s[0][t] = a; s[1][t] = b;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (t < o) { s[0][t] += s[0][t + o]; s[1][t] += s[1][t + o]; }
+    __syncthreads();
+  }
+  if (t == 0) { out[0] = s[0][0]; out[1] = s[1][0]; }
+}
+
+// ---------------------------------------------------------------------------
+// stage launch
+// ---------------------------------------------------------------------------
+template <int N, int GV, int MODE>
+static hipError_t launch_stage_t(sd2d_ctx *c, const StageArgs &a) {
+  constexpr size_t lds = stage_lds_bytes<N, GV>();
+  static_assert(lds <= 65536, "stage kernel LDS above 64 KiB");
+  const int grid = c->ntile * Lds<N, GV>::NSUB;
+  hipLaunchKernelGGL((stage_kernel<N, GV, MODE>), dim3(grid), dim3(STAGE_THREADS), lds,
+                     c->stream, a, c->ops, c->ph);
+  return hipGetLastError();
+}
+
+template <int N>
+static hipError_t launch_stage_n(sd2d_ctx *c, const StageArgs &a, int mode) {
+  const bool av = (mode & M_AV) != 0;
+  if (mode & M_COMPAT) {
+    return av ? launch_stage_t<N, 9, M_COMPAT | M_AV>(c, a)
+              : launch_stage_t<N, 9, M_COMPAT>(c, a);
+  }
+  const bool nr = (mode & M_NORMS) != 0;
+  if (av) return nr ? launch_stage_t<N, 2, M_RK | M_AV | M_NORMS>(c, a)
+                    : launch_stage_t<N, 2, M_RK | M_AV>(c, a);
+  return nr ? launch_stage_t<N, 2, M_RK | M_NORMS>(c, a) : launch_stage_t<N, 2, M_RK>(c, a);
+}
+
+static hipError_t launch_stage(sd2d_ctx *c, const StageArgs &a, int mode) {
+  switch (c->N) {
+    case 3: return launch_stage_n<3>(c, a, mode);
+    case 4: return launch_stage_n<4>(c, a, mode);
+    default: return launch_stage_n<5>(c, a, mode);
+  }
+}
+
+static StageArgs base_args(sd2d_ctx *c) {
+  StageArgs a{};
+  a.met = c->met;
+  a.ftab = c->ftab;
+  a.gmax = c->gmax;
+  a.slots = c->slots;
+  a.avtab = c->avtab;
+  a.ncell = c->ncell;
+  a.ntile = c->ntile;
+  a.sm_ind = c->sm_ind;
+  return a;
+}
+
+static int finalize_stage(sd2d_ctx *c) {
+  hipLaunchKernelGGL(k_finalize_max, dim3(1), dim3(NSLOT), 0, c->stream, c->slots, c->gmax);
+  HIPCHK(c, hipGetLastError());
+  return SD2D_OK;
+}
+
+extern "C" {
+
+const char *sd2d_last_error(const sd2d_ctx *ctx) {
+  return ctx ? ctx->err.c_str() : g_err.c_str();
+}
+
+int sd2d_create(const sd2d_mesh *m, const sd2d_params *p, sd2d_ctx **out) {
+  if (!out) return fail(nullptr, SD2D_EINVAL, "out is NULL");
+  *out = nullptr;
+  if (!m || !p) return fail(nullptr, SD2D_EINVAL, "mesh or params is NULL");
+  if (m->N < 3 || m->N > 5)
+    return fail(nullptr, SD2D_EUNSUPPORTED, "N must be 3, 4 or 5");
+  if (m->ncell <= 0 || m->nface <= 0 || !m->if2c || !m->lmat || !m->mmat || !m->jac ||
+      !m->s1 || !m->s2 || !m->xs)
+    return fail(nullptr, SD2D_EINVAL, "incomplete mesh");
+  if (p->k_stage != 3 && p->k_stage != 5)
+    return fail(nullptr, SD2D_EINVAL, "k_stage can be only 3 or 5");
+  if (p->vismode == 1 && (p->mu != 0.0 || p->eta0 != 0.0))
+    return fail(nullptr, SD2D_EUNSUPPORTED,
+                "viscous/resistive fluxes (mu, eta0 != 0) are not implemented on the device");
+  if (p->precision != SD2D_FP64)
+    return fail(nullptr, SD2D_EUNSUPPORTED, "only SD2D_FP64 is implemented");
+  sd2d_ctx *c = new (std::nothrow) sd2d_ctx();
+  if (!c) return fail(nullptr, SD2D_ENOMEM, "host allocation failed");
+  HostTables tb;
+  {
+    std::string e;
+    const int r = build_host_tables(m, p, tb, e);
+    if (r) { delete c; return fail(nullptr, r, e); }
+  }
+  const int N = m->N, nc = m->ncell, CPW = tb.cpw, N2 = N * N;
+  c->N = N; c->ncell = nc; c->cpw = CPW; c->ntile = tb.ntile; c->p = *p;
+  c->ctime = 0.0; c->iter = 0;
+  c->ops = tb.ops;
+  c->ph = tb.ph;
+  const int nt = c->ntile;
+  const std::vector<int> &ft = tb.ft;
+  const std::vector<double> &met = tb.met;
+  const std::vector<double> &av = tb.av;
+
+  auto bail = [&](int code) { g_err = c->err; sd2d_destroy(c); return code; };
+#define CK(call)                                                                   \
+  do {                                                                             \
+    hipError_t e_ = (call);                                                        \
+    if (e_ != hipSuccess) {                                                        \
+      c->err = std::string(#call) + ": " + hipGetErrorString(e_);                  \
+      return bail(e_ == hipErrorOutOfMemory ? SD2D_ENOMEM : SD2D_EHIP);            \
+    }                                                                              \
+  } while (0)
+  CK(hipGetDevice(&c->device));
+  CK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  const size_t qn = (size_t)nt * 9 * N2 * CPW;
+  for (int i = 0; i < 3; ++i) CK(hipMalloc(&c->q[i], qn * sizeof(double)));
+  if (p->k_stage == 5)
+    for (int i = 0; i < 4; ++i) CK(hipMalloc(&c->fsr[i], qn * sizeof(double)));
+  CK(hipMalloc(&c->met, met.size() * sizeof(double)));
+  CK(hipMalloc(&c->ftab, ft.size() * sizeof(int)));
+  CK(hipMalloc(&c->avtab, av.size() * sizeof(double)));
+  CK(hipMalloc(&c->gmax, 2 * sizeof(double)));
+  CK(hipMalloc(&c->slots, 2 * NSLOT * sizeof(unsigned long long)));
+  CK(hipMalloc(&c->sm_ind, 2 * (size_t)nc * sizeof(double)));
+  CK(hipMalloc(&c->normbuf, 2 * (size_t)nt * sizeof(double)));
+  CK(hipMalloc(&c->normout, 2 * sizeof(double)));
+  CK(hipMalloc(&c->stagef, (size_t)nc * 9 * N2 * sizeof(double)));
+  CK(hipMemcpyAsync(c->met, met.data(), met.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  CK(hipMemcpyAsync(c->ftab, ft.data(), ft.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  CK(hipMemcpyAsync(c->avtab, av.data(), av.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  for (int i = 0; i < 3; ++i) CK(hipMemsetAsync(c->q[i], 0, qn * sizeof(double), c->stream));
+  CK(hipMemsetAsync(c->sm_ind, 0, 2 * (size_t)nc * sizeof(double), c->stream));
+  hipLaunchKernelGGL(k_set_max, dim3(1), dim3(NSLOT), 0, c->stream, c->slots, c->gmax, 0.0, 0.0,
+                     0.0, 0.0);
+  CK(hipGetLastError());
+  CK(hipStreamSynchronize(c->stream));
+#undef CK
+  *out = c;
+  return SD2D_OK;
+}
+
+void sd2d_destroy(sd2d_ctx *c) {
+  if (!c) return;
+  if (c->stream) hipStreamSynchronize(c->stream);
+  for (double *b : c->q) if (b) hipFree(b);
+  for (double *b : c->fsr) if (b) hipFree(b);
+  void *ptrs[] = {c->met, c->ftab, c->avtab, c->gmax, c->slots, c->sm_ind, c->normbuf,
+                  c->normout, c->stagef, c->nablaf};
+  for (void *b : ptrs) if (b) hipFree(b);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+}
+
+static int f2t(sd2d_ctx *c, const double *dev_f, double *dev_t) {
+  const int blocks = 1024;
+  switch (c->N) {
+    case 3: hipLaunchKernelGGL(k_f2t<3>, dim3(blocks), dim3(256), 0, c->stream, dev_f, dev_t, c->ncell, c->ntile); break;
+    case 4: hipLaunchKernelGGL(k_f2t<4>, dim3(blocks), dim3(256), 0, c->stream, dev_f, dev_t, c->ncell, c->ntile); break;
+    default: hipLaunchKernelGGL(k_f2t<5>, dim3(blocks), dim3(256), 0, c->stream, dev_f, dev_t, c->ncell, c->ntile); break;
+  }
+  HIPCHK(c, hipGetLastError());
+  return SD2D_OK;
+}
+
+static int t2f(sd2d_ctx *c, const double *dev_t, double *dev_f) {
+  const int blocks = 1024;
+  switch (c->N) {
+    case 3: hipLaunchKernelGGL(k_t2f<3>, dim3(blocks), dim3(256), 0, c->stream, dev_t, dev_f, c->ncell); break;
+    case 4: hipLaunchKernelGGL(k_t2f<4>, dim3(blocks), dim3(256), 0, c->stream, dev_t, dev_f, c->ncell); break;
+    default: hipLaunchKernelGGL(k_t2f<5>, dim3(blocks), dim3(256), 0, c->stream, dev_t, dev_f, c->ncell); break;
+  }
+  HIPCHK(c, hipGetLastError());
+  return SD2D_OK;
+}
+
+int sd2d_set_state(sd2d_ctx *c, const double *q, const sd2d_maxima *mx) {
+  if (!c) return fail(nullptr, SD2D_EINVAL, "ctx is NULL");
+  const size_t n = (size_t)c->ncell * 9 * c->N * c->N;
+  if (q) {
+    HIPCHK(c, hipMemcpyAsync(c->stagef, q, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    int r = f2t(c, c->stagef, c->q[c->cur]);
+    if (r) return r;
+  }
+  if (mx) {
+    hipLaunchKernelGGL(k_set_max, dim3(1), dim3(NSLOT), 0, c->stream, c->slots, c->gmax,
+                       mx->eigvmax, mx->umax, mx->eigvmax_g, mx->umax_g);
+    HIPCHK(c, hipGetLastError());
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return SD2D_OK;
+}
+
+int sd2d_get_state(sd2d_ctx *c, double *q, sd2d_maxima *mx) {
+  if (!c) return fail(nullptr, SD2D_EINVAL, "ctx is NULL");
+  const size_t n = (size_t)c->ncell * 9 * c->N * c->N;
+  if (q) {
+    int r = t2f(c, c->q[c->cur], c->stagef);
+    if (r) return r;
+    HIPCHK(c, hipMemcpyAsync(q, c->stagef, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  }
+  if (mx) {
+    unsigned long long s[2 * NSLOT];
+    double g[2];
+    HIPCHK(c, hipMemcpyAsync(s, c->slots, sizeof(s), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(g, c->gmax, sizeof(g), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    unsigned long long e = 0, u = 0;
+    for (int i = 0; i < NSLOT; ++i) { e = std::max(e, s[2 * i]); u = std::max(u, s[2 * i + 1]); }
+    long long ei = (long long)e, ui = (long long)u;
+    std::memcpy(&mx->eigvmax, &ei, 8);
+    std::memcpy(&mx->umax, &ui, 8);
+    mx->eigvmax_g = g[0]; mx->umax_g = g[1];
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return SD2D_OK;
+}
+
+int sd2d_set_time(sd2d_ctx *c, double ctime, int32_t iter) {
+  if (!c) return fail(nullptr, SD2D_EINVAL, "ctx is NULL");
+  c->ctime = ctime; c->iter = iter;
+  return SD2D_OK;
+}
+
+static int read_c_h(sd2d_ctx *c) {
+  double g[2];
+  HIPCHK(c, hipMemcpyAsync(g, c->gmax, sizeof(g), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->last_c_h = std::sqrt(g[0] * (g[0] - g[1]));
+  return SD2D_OK;
+}
+
+int sd2d_compresid(sd2d_ctx *c, double *resid_out, double *nablaqs_out) {
+  if (!c) return fail(nullptr, SD2D_EINVAL, "ctx is NULL");
+  const size_t n = (size_t)c->ncell * 9 * c->N * c->N;
+  if (!c->nablaf) HIPCHK(c, hipMalloc(&c->nablaf, 2 * n * sizeof(double)));
+  int r = read_c_h(c);
+  if (r) return r;
+  StageArgs a = base_args(c);
+  a.qin = c->q[c->cur];
+  a.resid_f = c->stagef;
+  a.nabla_f = c->nablaf;
+  const int mode = M_COMPAT | (c->p.artiv == 1 && c->p.vismode == 1 ? M_AV : 0);
+  HIPCHK(c, launch_stage(c, a, mode));
+  c->have_av = (mode & M_AV) != 0;
+  r = finalize_stage(c);
+  if (r) return r;
+  if (resid_out)
+    HIPCHK(c, hipMemcpyAsync(resid_out, c->stagef, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (nablaqs_out)
+    HIPCHK(c, hipMemcpyAsync(nablaqs_out, c->nablaf, 2 * n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return SD2D_OK;
+}
+
+int sd2d_compresid_q(sd2d_ctx *c, const double *q, double *resid_out, double *nablaqs_out) {
+  int r = sd2d_set_state(c, q, nullptr);
+  if (r) return r;
+  return sd2d_compresid(c, resid_out, nablaqs_out);
+}
+
+static int step_rk(sd2d_ctx *c, bool norms) {
+  RkBuffers rb{{c->q[0], c->q[1], c->q[2]}, {c->fsr[0], c->fsr[1], c->fsr[2]}, c->cur};
+  const int av = (c->p.artiv == 1 && c->p.vismode == 1) ? M_AV : 0;
+  auto launch = [&](StageArgs &a, int mode) -> int {
+    if (mode & M_NORMS) a.normbuf = c->normbuf;
+    HIPCHK(c, launch_stage(c, a, mode));
+    return finalize_stage(c);
+  };
+  const int r = (c->p.k_stage == 5) ? fsrk_sequence(rb, c->p.dt, norms, av, base_args(c), launch)
+                                    : s5p3_sequence(rb, c->p.dt, norms, av, base_args(c), launch);
+  if (r) return r;
+  c->cur = rb.cur;
+  // ctime bookkeeping of RK_S5P3 / FiveStageRk: only the final value
+  // (ctime0 + dt) survives a step
+  c->ctime = c->ctime + c->p.dt;
+  return SD2D_OK;
+}
+
+int sd2d_step(sd2d_ctx *c, int32_t nsteps, int32_t norm_every) {
+  if (!c) return fail(nullptr, SD2D_EINVAL, "ctx is NULL");
+  if (nsteps < 0) return fail(c, SD2D_EINVAL, "nsteps < 0");
+  for (int s = 0; s < nsteps; ++s) {
+    c->iter += 1;
+    const bool norms = norm_every > 0 && (c->iter % norm_every) == 1;
+    int r = step_rk(c, norms);
+    if (r) return r;
+    c->have_av = c->p.artiv == 1 && c->p.vismode == 1;
+    if (norms) {
+      hipLaunchKernelGGL(k_norm_reduce, dim3(1), dim3(256), 0, c->stream, c->normbuf, c->ntile,
+                         c->normout);
+      HIPCHK(c, hipGetLastError());
+      double s2[2];
+      HIPCHK(c, hipMemcpyAsync(s2, c->normout, sizeof(s2), hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+      const double nn = (double)c->p.ncellglob * c->N * c->N;
+      c->resid_rms = std::sqrt(s2[0] / (nn * 9.0));
+      c->divb_rms = std::sqrt(s2[1] / nn);
+      c->norms_valid = 1;
+    }
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return SD2D_OK;
+}
+
+int sd2d_get_diag(sd2d_ctx *c, sd2d_diag *d) {
+  if (!c || !d) return fail(c, SD2D_EINVAL, "NULL argument");
+  double g[2];
+  HIPCHK(c, hipMemcpyAsync(g, c->gmax, sizeof(g), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  d->resid_rms = c->resid_rms;
+  d->divb_rms = c->divb_rms;
+  d->eigvmax_g = g[0];
+  d->umax_g = g[1];
+  d->c_h = c->last_c_h;
+  const double h = std::sqrt(c->p.dxcycl * c->p.dycycl / (double)c->p.ncellglob);
+  d->epsilon0 = g[0] * h / (double)(c->N - 1) * 0.05;
+  d->ctime = c->ctime;
+  d->iter = c->iter;
+  d->norms_valid = c->norms_valid;
+  return SD2D_OK;
+}
+
+int sd2d_get_av(sd2d_ctx *c, double *sm_ind, double *muav, double *epsilon0) {
+  if (!c) return fail(nullptr, SD2D_EINVAL, "ctx is NULL");
+  if (c->p.artiv != 1 || c->p.vismode != 1)
+    return fail(c, SD2D_EINVAL, "the AV sensor runs only with vismode=1 and ARTIV=1");
+  std::vector<double> sm((size_t)2 * c->ncell);
+  double g[2];
+  HIPCHK(c, hipMemcpyAsync(sm.data(), c->sm_ind, sm.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(g, c->gmax, sizeof(g), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  // AV.f90:106-140
+  const double h = std::sqrt(c->p.dxcycl * c->p.dycycl / (double)c->p.ncellglob);
+  const double eps = g[0] * h / (double)(c->N - 1) * 0.05;
+  if (epsilon0) *epsilon0 = eps;
+  if (sm_ind) std::memcpy(sm_ind, sm.data(), sm.size() * sizeof(double));
+  if (muav) {
+    const double lo[2] = {-5.0, -3.0}, hi[2] = {0.0, 1.0};
+    const double pi = 3.141592653589793;
+    for (int ic = 0; ic < c->ncell; ++ic)
+      for (int v = 0; v < 2; ++v) {
+        const double S = sm[2 * (size_t)ic + v];
+        const double k = (hi[v] - lo[v]) * 0.5, s0 = (hi[v] + lo[v]) * 0.5;
+        double mu;
+        if (S < lo[v]) mu = 0.0;
+        else if (S > hi[v]) mu = eps;
+        else mu = eps / 2.0 * (1.0 + std::sin(pi * (S - s0) / 2.0 / k));
+        muav[2 * (size_t)ic + v] = mu;
+      }
+  }
+  return SD2D_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,209 @@
+// sd2d_host.hpp - host-side set-up of the device tables (no HIP calls):
+// the per-cell face table, the tile-major metrics and the AV projections,
+// built once from the arrays module setup2d holds after INIT_SETUP.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/sd2d.h"
+#include "sd2d_device.hpp"
+
+namespace sd2d {
+
+struct HostTables {
+  int N = 0, ncell = 0, cpw = 0, ntile = 0;
+  Ops ops{};
+  Phys ph{};
+  std::vector<int> ft;       // [tile][k][{neighbour base, info}][ci]
+  std::vector<double> met;   // [tile][metric slot][ci]
+  std::vector<double> av;    // hlow[isl][is], then hhigh[is][isl]
+};
+
+namespace hostdetail {
+
+struct HostMesh {
+  int N, nc, nface;
+  const sd2d_mesh *m;
+  int if2c(int f, int k) const { return m->if2c[f + (size_t)nface * k]; }
+  // own metric normal of local face k (0-based) at own point nfp (1-based):
+  // S1(1,:) on faces 2/4, S2(2,:) on faces 1/3 (interfaceflux_all.f90:47,54)
+  void normal(int cell, int k, int nfp, double &nx, double &ny) const {
+    int ifp, jfp;  // 1-based, face2fpconnec (initsetup.f90:356-380)
+    switch (k) {
+      case 0: ifp = nfp; jfp = 1; break;
+      case 1: ifp = N + 1; jfp = nfp; break;
+      case 2: ifp = N - nfp + 1; jfp = N + 1; break;
+      default: ifp = 1; jfp = N - nfp + 1; break;
+    }
+    if (k == 1 || k == 3) {
+      const size_t b = 2 * (2 * ((ifp - 1) + (size_t)(N + 1) * ((jfp - 1) + (size_t)N * cell)));
+      nx = m->s1[b + 0]; ny = m->s1[b + 2];
+    } else {
+      const size_t b = 2 * (2 * ((ifp - 1) + (size_t)N * ((jfp - 1) + (size_t)(N + 1) * cell)));
+      nx = m->s2[b + 1]; ny = m->s2[b + 3];
+    }
+  }
+};
+
+template <typename Tm>
+inline void put_tile(std::vector<Tm> &dst, int NM, int CPW, int cell, int slot, Tm v) {
+  const size_t tile = cell / CPW, ci = cell % CPW;
+  dst[(tile * NM + slot) * CPW + ci] = v;
+}
+
+inline double lagrange(const double *x, int n, int i, double t) {
+  double a = 1.0, b = 1.0;
+  for (int s = 0; s < n; ++s)
+    if (s != i) { a *= (t - x[s]); b *= (x[i] - x[s]); }
+  return a / b;
+}
+
+}  // namespace hostdetail
+
+// Per-cell face table and the check that the left cell's normal equals +-
+// the right cell's own metric vector bit for bit at every face point.
+inline int build_face_table(const sd2d_mesh *m, int CPW, std::vector<int> &nb,
+                            std::vector<int> &info, std::string &err) {
+  using hostdetail::HostMesh;
+  const int N = m->N, nc = m->ncell, N2 = N * N;
+  HostMesh hm{N, nc, m->nface, m};
+  nb.assign((size_t)nc * 4, 0);
+  info.assign((size_t)nc * 4, -1);
+  auto chk_face = [&](int f) { return f >= 1 && f <= m->nface; };
+  auto base_of = [&](int cell) {
+    const size_t tile = cell / CPW, ci = cell % CPW;
+    return (long long)(tile * 9 * N2 * CPW + ci);
+  };
+  auto bad = [&](int code, const char *msg) { err = msg; return code; };
+  auto pair = [&](int cl, int kl, int cr, int kr) -> int {
+    // kl, kr 1-based local faces; cl, cr 0-based cells
+    if (cl < 0 || cl >= nc || cr < 0 || cr >= nc || kl < 1 || kl > 4 || kr < 1 || kr > 4)
+      return bad(SD2D_EINVAL, "face table entry out of range");
+    int sneg = -1;
+    for (int nfp = 1; nfp <= N; ++nfp) {
+      double lx, ly, rx, ry;
+      hm.normal(cl, kl - 1, nfp, lx, ly);
+      hm.normal(cr, kr - 1, N - nfp + 1, rx, ry);
+      int s;
+      if (lx == rx && ly == ry) s = 0;
+      else if (lx == -rx && ly == -ry) s = 1;
+      else
+        return bad(SD2D_EUNSUPPORTED,
+                   "face normals of neighbouring cells differ beyond sign "
+                   "(non-conforming or curved faces are not supported)");
+      if (sneg >= 0 && s != sneg)
+        return bad(SD2D_EUNSUPPORTED, "face normal sign varies along a face");
+      sneg = s;
+    }
+    const long long bl = base_of(cl), br = base_of(cr);
+    if (br > 0x7fffffffLL || bl > 0x7fffffffLL)
+      return bad(SD2D_EUNSUPPORTED, "mesh too large for 32-bit offsets");
+    nb[4 * (size_t)cl + (kl - 1)] = (int)br;
+    info[4 * (size_t)cl + (kl - 1)] = finfo_pack(kr - 1, 1, FT_INTERIOR, 0);
+    nb[4 * (size_t)cr + (kr - 1)] = (int)bl;
+    info[4 * (size_t)cr + (kr - 1)] = finfo_pack(kl - 1, 0, FT_INTERIOR, sneg);
+    return SD2D_OK;
+  };
+  for (int i = 0; i < m->ninter; ++i) {
+    const int f = m->ibfinter[i];
+    if (!chk_face(f)) return bad(SD2D_EINVAL, "IBFINTER entry out of range");
+    const int r = pair(hm.if2c(f - 1, 0) - 1, hm.if2c(f - 1, 2), hm.if2c(f - 1, 1) - 1,
+                       hm.if2c(f - 1, 3));
+    if (r) return r;
+  }
+  for (int i = 0; i < m->npair; ++i) {   // cyclocflux_all.f90:15-24
+    const int k1 = m->cyc2fpair[i], k2 = m->cyc2fpair[i + m->npair];
+    if (!chk_face(k1) || !chk_face(k2)) return bad(SD2D_EINVAL, "CYC2FPAIR out of range");
+    const int r = pair(hm.if2c(k1 - 1, 0) - 1, hm.if2c(k1 - 1, 2), hm.if2c(k2 - 1, 0) - 1,
+                       hm.if2c(k2 - 1, 2));
+    if (r) return r;
+  }
+  // boundary lists in BCFLUX order (bcflux.f90: INLE, OUTL, WALL, SYMP, FREE)
+  struct L { int n; const int32_t *f; int bc; } lists[5] = {
+      {m->ninlet, m->ibfinl, FT_INLE}, {m->noutlet, m->ibfout, FT_OUTL},
+      {m->nwall, m->ibfwal, FT_WALL}, {m->nsymp, m->ibfsymp, FT_SYMP},
+      {m->nfree, m->ibfree, FT_FREE}};
+  for (auto &l : lists)
+    for (int i = 0; i < l.n; ++i) {
+      const int f = l.f[i];
+      if (!chk_face(f)) return bad(SD2D_EINVAL, "boundary list entry out of range");
+      const int cl = hm.if2c(f - 1, 0) - 1, kl = hm.if2c(f - 1, 2);
+      if (cl < 0 || cl >= nc || kl < 1 || kl > 4)
+        return bad(SD2D_EINVAL, "boundary face table entry out of range");
+      nb[4 * (size_t)cl + (kl - 1)] = 0;
+      info[4 * (size_t)cl + (kl - 1)] = finfo_pack(0, 1, l.bc, 0);
+    }
+  for (size_t i = 0; i < info.size(); ++i)
+    if (info[i] < 0)
+      return bad(SD2D_EINVAL,
+                 "a cell face is neither interior, periodic nor on a boundary list "
+                 "(it would be a processor interface)");
+  return SD2D_OK;
+}
+
+// Everything the kernels read besides the state.
+inline int build_host_tables(const sd2d_mesh *m, const sd2d_params *p, HostTables &t,
+                             std::string &err) {
+  using hostdetail::put_tile;
+  const int N = m->N, nc = m->ncell, CPW = cpw_of(N), N2 = N * N;
+  t.N = N; t.ncell = nc; t.cpw = CPW; t.ntile = (nc + CPW - 1) / CPW;
+  for (int i = 0; i <= N; ++i)
+    for (int s = 0; s < N; ++s) {
+      t.ops.L[i][s] = m->lmat[i + (N + 1) * s];
+      t.ops.M[i][s] = m->mmat[i + (N + 1) * s];
+    }
+  t.ph.alpha = p->alpha; t.ph.rinf = p->rinf; t.ph.pinf = p->pinf; t.ph.uinf = p->uinf;
+  t.ph.vinf = p->vinf; t.ph.bxinf = p->bxinf; t.ph.byinf = p->byinf;
+  t.ph.vismode = p->vismode;
+
+  std::vector<int> nb, info;
+  const int r = build_face_table(m, CPW, nb, info, err);
+  if (r) return r;
+  const int nt = t.ntile;
+  t.ft.assign((size_t)nt * 8 * CPW, 0);
+  for (int cell = 0; cell < nc; ++cell)
+    for (int k = 0; k < 4; ++k) {
+      put_tile<int>(t.ft, 8, CPW, cell, 2 * k, nb[4 * (size_t)cell + k]);
+      put_tile<int>(t.ft, 8, CPW, cell, 2 * k + 1, info[4 * (size_t)cell + k]);
+    }
+  // padded cells: FREE boundary so that nothing is dereferenced for them
+  for (int cell = nc; cell < nt * CPW; ++cell)
+    for (int k = 0; k < 4; ++k)
+      put_tile<int>(t.ft, 8, CPW, cell, 2 * k + 1, finfo_pack(0, 1, FT_FREE, 0));
+
+  // metrics: S1(1,1:2,rfp,js), S2(2,1:2,is,rfp), 1/Jac(is,js)
+  const int NM = 4 * N * (N + 1) + N2;
+  t.met.assign((size_t)nt * NM * CPW, 0.0);
+  for (int cell = 0; cell < nc; ++cell) {
+    for (int js = 0; js < N; ++js)
+      for (int rfp = 0; rfp <= N; ++rfp) {
+        const size_t b = 2 * (2 * (rfp + (size_t)(N + 1) * (js + (size_t)N * cell)));
+        put_tile<double>(t.met, NM, CPW, cell, 0 + rfp + (N + 1) * js, m->s1[b + 0]);
+        put_tile<double>(t.met, NM, CPW, cell, N * (N + 1) + rfp + (N + 1) * js, m->s1[b + 2]);
+      }
+    for (int rfp = 0; rfp <= N; ++rfp)
+      for (int is = 0; is < N; ++is) {
+        const size_t b = 2 * (2 * (is + (size_t)N * (rfp + (size_t)(N + 1) * cell)));
+        put_tile<double>(t.met, NM, CPW, cell, 2 * N * (N + 1) + rfp + (N + 1) * is, m->s2[b + 1]);
+        put_tile<double>(t.met, NM, CPW, cell, 3 * N * (N + 1) + rfp + (N + 1) * is, m->s2[b + 3]);
+      }
+    for (int js = 0; js < N; ++js)
+      for (int is = 0; is < N; ++is)
+        put_tile<double>(t.met, NM, CPW, cell, 4 * N * (N + 1) + js + N * is,
+                         1.0 / m->jac[is + N * (js + (size_t)N * cell)]);
+  }
+  // AV projections (AV.f90:18-63): hlow[isl][is], hhigh[is][isl]
+  t.av.assign((size_t)2 * N * (N - 1), 0.0);
+  if (m->xs_low) {
+    for (int il = 0; il < N - 1; ++il)
+      for (int is = 0; is < N; ++is)
+        t.av[il * N + is] = hostdetail::lagrange(m->xs, N, is, m->xs_low[il]);
+    for (int is = 0; is < N; ++is)
+      for (int il = 0; il < N - 1; ++il)
+        t.av[(N - 1) * N + is * (N - 1) + il] = hostdetail::lagrange(m->xs_low, N - 1, il, m->xs[is]);
+  }
+  return SD2D_OK;
+}
+
+}  // namespace sd2d

@@ -1,0 +1,134 @@
+"""Python host over the C ABI of libsd2d.so (include/sd2d.h).
+
+Mirrors the reference's interface for this path: `compresid()` has the
+per-call contract of `subroutine compresid` (compresid.f90:1-80), `step()`
+advances like RK_S5P3 / FiveStageRk called from `iterations`
+(iterations.f90:19-31), and the state and running maxima move in and out in
+the reference's Fortran layouts.  There is no fallback: if the device
+library is missing or fails, the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+from . import mesh as M
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "lib", "libsd2d.so")
+_lib = None
+
+EXPORTS = ("sd2d_create", "sd2d_destroy", "sd2d_last_error", "sd2d_set_state",
+           "sd2d_get_state", "sd2d_set_time", "sd2d_compresid",
+           "sd2d_compresid_q", "sd2d_step", "sd2d_get_diag", "sd2d_get_av")
+
+
+class Sd2dError(RuntimeError):
+    pass
+
+
+def load_library(path: str = LIBPATH):
+    """Load libsd2d.so and declare the ABI; raises if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise Sd2dError(f"{path} is not built; run `python -m sddc_amd.build`")
+    L = C.CDLL(path)
+    vp = C.c_void_p
+    L.sd2d_create.argtypes = [C.POINTER(abi.Sd2dMesh), C.POINTER(abi.Sd2dParams),
+                              C.POINTER(vp)]
+    L.sd2d_destroy.argtypes = [vp]
+    L.sd2d_destroy.restype = None
+    L.sd2d_last_error.argtypes = [vp]
+    L.sd2d_last_error.restype = C.c_char_p
+    L.sd2d_set_state.argtypes = [vp, vp, C.POINTER(abi.Sd2dMaxima)]
+    L.sd2d_get_state.argtypes = [vp, vp, C.POINTER(abi.Sd2dMaxima)]
+    L.sd2d_set_time.argtypes = [vp, C.c_double, C.c_int32]
+    L.sd2d_compresid.argtypes = [vp, vp, vp]
+    L.sd2d_compresid_q.argtypes = [vp, vp, vp, vp]
+    L.sd2d_step.argtypes = [vp, C.c_int32, C.c_int32]
+    L.sd2d_get_diag.argtypes = [vp, C.POINTER(abi.Sd2dDiag)]
+    L.sd2d_get_av.argtypes = [vp, vp, vp, C.POINTER(C.c_double)]
+    for f in EXPORTS:
+        if f not in ("sd2d_destroy", "sd2d_last_error"):
+            getattr(L, f).restype = C.c_int
+    _lib = L
+    return L
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class Sd2d:
+    """One device context for one mesh (one GPU)."""
+
+    def __init__(self, case: M.Case, precision: int = abi.SD2D_FP64):
+        self.L = load_library()
+        self.case = case
+        self.N, self.nc = case.N, case.ncell
+        ms, self._keep = abi.mesh_struct(case)
+        ps = abi.params_struct(case.params, case.ncell, precision)
+        h = C.c_void_p()
+        rc = self.L.sd2d_create(C.byref(ms), C.byref(ps), C.byref(h))
+        if rc != 0:
+            raise Sd2dError(f"sd2d_create: {rc}: {self.L.sd2d_last_error(None).decode()}")
+        self.h = h
+        self._keep = None   # tables were copied by sd2d_create
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise Sd2dError(f"{what}: {rc}: {self.L.sd2d_last_error(self.h).decode()}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.sd2d_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- state -------------------------------------------------------------
+    def set_state(self, Q, maxima=None):
+        Q = np.asfortranarray(Q, dtype=np.float64)
+        assert Q.shape == (9, self.N, self.N, self.nc)
+        mx = abi.Sd2dMaxima(*[float(x) for x in maxima]) if maxima is not None else None
+        self._check(self.L.sd2d_set_state(self.h, _ptr(Q), C.byref(mx) if mx else None),
+                    "sd2d_set_state")
+
+    def get_state(self):
+        Q = np.zeros((9, self.N, self.N, self.nc), order="F")
+        mx = abi.Sd2dMaxima()
+        self._check(self.L.sd2d_get_state(self.h, _ptr(Q), C.byref(mx)), "sd2d_get_state")
+        return Q, np.array([mx.eigvmax, mx.umax, mx.eigvmax_g, mx.umax_g])
+
+    def set_time(self, ctime, it=0):
+        self._check(self.L.sd2d_set_time(self.h, float(ctime), int(it)), "sd2d_set_time")
+
+    # -- the path ----------------------------------------------------------
+    def compresid(self, want_resid=True, want_nabla=True):
+        r = np.zeros((9, self.N, self.N, self.nc), order="F") if want_resid else None
+        g = np.zeros((9, 2, self.N, self.N, self.nc), order="F") if want_nabla else None
+        self._check(self.L.sd2d_compresid(self.h, _ptr(r), _ptr(g)), "sd2d_compresid")
+        return r, g
+
+    def step(self, nsteps, norm_every=0):
+        self._check(self.L.sd2d_step(self.h, int(nsteps), int(norm_every)), "sd2d_step")
+
+    def diag(self):
+        d = abi.Sd2dDiag()
+        self._check(self.L.sd2d_get_diag(self.h, C.byref(d)), "sd2d_get_diag")
+        return {k: getattr(d, k) for k, _ in abi.Sd2dDiag._fields_}
+
+    def get_av(self):
+        s = np.zeros((2, self.nc), order="F"); m = np.zeros((2, self.nc), order="F")
+        e = C.c_double()
+        self._check(self.L.sd2d_get_av(self.h, _ptr(s), _ptr(m), C.byref(e)), "sd2d_get_av")
+        return s, m, e.value
