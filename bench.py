@@ -1,0 +1,146 @@
+#!/usr/bin/env python3
+"""Benchmark of the SD2D-MHD residual + RK path on one MI355X.
+
+Workload: the 512 x 1024-quad cylinder O-grid, r in [0.5, 30], p=4 (N=5,
+25 solution points per cell, 13.1M solution points x 9 variables), FP64,
+RK_S5P3 (5 residual evaluations per step), AV sensor on, residual/divB
+norms every 10 steps as the reference's time loop does.  One step is one
+RK_S5P3 step over the whole mesh with the state resident in HBM.
+
+Prints one JSON line:
+  value       DOF-stage updates/s = NDOF * 5 * steps / wall time
+  roofline    the fused stage kernel against HBM bandwidth: algorithmic
+              bytes per launch (NDOF * B_alg) / average launch time from
+              HIP events on the library's stream
+  cpu_baseline  the CPU restatement of the reference (OpenMP) on one step
+              of the same workload on this host's cores
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "DOF-stage updates/s, FP64 plain vs mixed at stated L2 error; % HBM peak"
+
+
+def build_workload(name: str):
+    from sddc_amd import cases
+    if name == "c2":
+        return cases.c2_cylinder(), "cylinder O-grid 512x1024 quads, r 0.5-30, p=4 (N=5), FP64, RK_S5P3"
+    if name == "c1":
+        return cases.c1_cylinder(), "cylinder O-grid 32x64 quads, r 0.5-15, p=3 (N=4), FP64, RK_S5P3"
+    raise SystemExit(f"unknown workload {name}")
+
+
+def traffic_from_profiles(kernel_tag: str):
+    """HBM bytes per launch of the stage kernel measured with rocprofv3
+    --pmc (FETCH_SIZE, WRITE_SIZE), scaled by the copy-kernel calibration,
+    as committed under profiles/; None when not measured."""
+    p = os.path.join(HERE, "profiles", "pmc_traffic.json")
+    if not os.path.exists(p):
+        return None
+    try:
+        d = json.load(open(p))
+        return d.get(kernel_tag, {}).get("bytes_per_launch")
+    except Exception:
+        return None
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2"])
+    ap.add_argument("--cpu-steps", type=int, default=1,
+                    help="steps of the CPU baseline sample (0 = skip)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        sys.stderr.write("bench.py measures one GPU only: this path runs the whole mesh on "
+                         "one MI355X (multi-GPU sharding is a design, not built); "
+                         "run with --gpus 1\n")
+        return 2
+
+    from sddc_amd import cases
+    from sddc_amd.solver import Sd2d
+
+    t = time.perf_counter()
+    (case, Q0), desc = build_workload(args.workload)
+    print(f"[bench] workload {args.workload}: {case.ncell} cells, N={case.N}, "
+          f"built in {time.perf_counter() - t:.1f}s", file=sys.stderr, flush=True)
+    N = case.N
+    ndof = cases.dof(case)
+    g = Sd2d(case)
+    g.set_state(Q0, [0.0, 0.0, 0.0, 0.0])
+    g.step(args.warmup, 10)
+    print(f"[bench] warmup {args.warmup} steps done", file=sys.stderr, flush=True)
+
+    t0 = time.perf_counter()
+    g.step(args.steps, 10)                 # returns after the device work is done
+    t1 = time.perf_counter()
+    wall = t1 - t0
+    value = ndof * 5 * args.steps / wall
+    print(f"[bench] timed {args.steps} steps in {wall:.4f}s", file=sys.stderr, flush=True)
+
+    # stage-kernel time from HIP events on the library's stream
+    g.set_profiling(True)
+    g.step(args.steps, 10)
+    ms, nlaunch = g.get_profile()
+    g.set_profiling(False)
+    avg_s = ms / 1e3 / max(nlaunch, 1)
+    balg = cases.alg_bytes_per_dof_stage(N, case.params.k_stage)
+    achieved = ndof * balg / avg_s / 1e9
+    d = g.diag()
+
+    res = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "DOF-stage/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded free stream with 1e-3 density perturbation)",
+        "config": {"workload": desc, "ncell": case.ncell, "N": N, "dof": ndof,
+                   "dt": case.params.dt, "rk": "RK_S5P3", "av": True,
+                   "norm_every": 10},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic_from_profiles("stage_kernel_N5_rk"),
+                     "bytes_per_dof_stage": balg, "stage_kernel_ms": avg_s * 1e3,
+                     "launches": nlaunch},
+        "diag": {"resid_rms": d["resid_rms"], "divb_rms": d["divb_rms"],
+                 "eigvmax_g": d["eigvmax_g"]},
+    }
+    g.close()
+
+    if args.cpu_steps > 0:
+        from oracle.oracle import Oracle, threads
+        o = Oracle(case)
+        o.set_state(Q0, [0.0, 0.0, 0.0, 0.0])
+        tc = time.perf_counter()
+        o.step(args.cpu_steps)
+        tcpu = time.perf_counter() - tc
+        res["cpu_baseline"] = {"value": ndof * 5 * args.cpu_steps / tcpu,
+                               "unit": "DOF-stage/s", "cores": threads(), "kind": "port",
+                               "sample": f"{args.cpu_steps} RK_S5P3 step(s) of the same "
+                                         f"workload ({5 * args.cpu_steps} residual "
+                                         f"evaluations), {tcpu:.1f}s"}
+        o.close()
+    print(json.dumps(res), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

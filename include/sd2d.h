@@ -175,6 +175,16 @@ int sd2d_get_diag(sd2d_ctx *ctx, sd2d_diag *diag);
 int sd2d_get_av(sd2d_ctx *ctx, double *sm_ind, double *muav,
                 double *epsilon0);
 
+/*
+ * Measurement aid, no counterpart in the reference.  While profiling is on,
+ * every stage-kernel launch of sd2d_step / sd2d_compresid is bracketed by
+ * HIP events on the context's stream; sd2d_get_profile returns the summed
+ * stage-kernel time in milliseconds and the number of launches since
+ * profiling was last switched on.
+ */
+int sd2d_set_profiling(sd2d_ctx *ctx, int32_t on);
+int sd2d_get_profile(sd2d_ctx *ctx, double *stage_ms, int32_t *nlaunch);
+
 #ifdef __cplusplus
 }
 #endif

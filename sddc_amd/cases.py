@@ -1,0 +1,106 @@
+"""The workloads of this path, built from the mesh generators.
+
+  cylinder(...)     compressible MHD flow past an adiabatic cylinder on an
+                    O-grid (WALL at r0, INLE/OUTL far field)
+  orszag_tang(...)  periodic Orszag-Tang vortex on [0,2pi]^2
+
+The reference has no CFL logic; dt is fixed (readinput.f90:23-24).  Here
+it is computed once on the host from the initial state as
+dt = CFL * h_min / ((|u| + c_f)_max * (2p+1)), with h_min the shortest
+cell edge and c_f the fast magnetosonic speed, and then kept fixed.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from . import mesh as M
+
+
+def fast_speed_max(Q: np.ndarray, gam: float = M.GAM) -> float:
+    """max over points of |u| + c_f (fast magnetosonic speed)."""
+    rho = Q[0]
+    u = Q[1] / rho; v = Q[2] / rho; w = Q[3] / rho
+    B2 = Q[5] ** 2 + Q[6] ** 2 + Q[7] ** 2
+    p = (gam - 1) * (Q[4] - 0.5 * rho * (u * u + v * v + w * w) - 0.5 * B2 - 0.5 * Q[8] ** 2)
+    a2 = gam * p / rho
+    b2 = B2 / rho
+    cf = np.sqrt(a2 + b2)          # upper bound of c_f in any direction
+    return float(np.max(np.sqrt(u * u + v * v + w * w) + cf))
+
+
+def min_edge(m: M.Mesh) -> float:
+    c = m.cells - 1
+    h = np.inf
+    for a, b in M.IVFACE:
+        d = np.hypot(m.xv[c[:, b]] - m.xv[c[:, a]], m.yv[c[:, b]] - m.yv[c[:, a]])
+        h = min(h, float(d.min()))
+    return h
+
+
+def cfl_dt(case: M.Case, Q: np.ndarray, cfl: float) -> float:
+    p = case.N - 1
+    return cfl * min_edge(case.mesh) / (fast_speed_max(Q) * (2 * p + 1))
+
+
+def cylinder(nr: int, ntheta: int, N: int, r1: float = 15.0, mach: float = 1.5,
+             alfven_mach: float | None = None, b_angle_deg: float = 0.0,
+             bmag: float = 0.3, rho_pert: float = 0.0, seed: int = 0,
+             cfl: float = 0.3, k_stage: int = 3, farfield: str = "split"):
+    """Cylinder case.  With alfven_mach given, |B| = mach * sqrt(rho) /
+    alfven_mach (c_s = 1) at angle b_angle_deg; otherwise |B| = bmag along x.
+    Returns (case, Q0)."""
+    gam = M.GAM
+    rinf, pinf = 1.0, 1.0 / gam
+    u = mach * math.sqrt(gam * pinf / rinf)
+    if alfven_mach is not None:
+        bmag = u * math.sqrt(rinf) / alfven_mach
+    th = math.radians(b_angle_deg)
+    p = M.RunParams(N=N, k_stage=k_stage, rinf=rinf, pinf=pinf, uinf=u, vinf=0.0,
+                    bxinf=bmag * math.cos(th), byinf=bmag * math.sin(th))
+    m = M.ogrid(nr, ntheta, 0.5, r1, farfield=farfield)
+    case = M.build_case(m, p)
+    Q0 = M.freestream_state(case, rho_pert=rho_pert, seed=seed)
+    case.params.dt = cfl_dt(case, Q0, cfl)
+    return case, Q0
+
+
+def c1_cylinder(**kw):
+    """32 x 64 O-grid, r in [0.5, 15], p=3, Mach 1.5, B=(0.3,0,0)."""
+    return cylinder(32, 64, 4, **kw)
+
+
+def c2_cylinder(**kw):
+    """512 x 1024 O-grid, r in [0.5, 30], p=4, Mach 1.5, Alfven Mach 4 at 30
+    degrees, seeded 1e-3 uniform density perturbation."""
+    kw.setdefault("seed", 2)
+    return cylinder(512, 1024, 5, r1=30.0, alfven_mach=4.0, b_angle_deg=30.0,
+                    rho_pert=1e-3, **kw)
+
+
+def orszag_tang(n: int, N: int = 4, rho_pert: float = 1e-4, seed: int = 3,
+                cfl: float = 0.3, k_stage: int = 3):
+    """Periodic Orszag-Tang vortex on n x n quads (C3 uses n=1024, p=3)."""
+    m = M.periodic_square(n, n)
+    p = M.RunParams(N=N, k_stage=k_stage, dxcycl=m.dxcycl, dycycl=m.dycycl,
+                    rinf=25.0 / 9.0, pinf=5.0 / 3.0, uinf=0.0, vinf=0.0,
+                    bxinf=0.0, byinf=0.0)
+    case = M.build_case(m, p)
+    Q0 = M.orszag_tang_state(case, rho_pert=rho_pert, seed=seed)
+    case.params.dt = cfl_dt(case, Q0, cfl)
+    return case, Q0
+
+
+def dof(case: M.Case) -> int:
+    return case.ncell * case.N * case.N
+
+
+def alg_bytes_per_dof_stage(N: int, k_stage: int = 3, metric_bytes: int = 8) -> float:
+    """Compulsory HBM bytes per DOF-stage of a fused stage: the RK state
+    traffic (RK_S5P3: 13 state passes of 72 B over 5 stages; FiveStageRk:
+    18 + the stored residual) plus Jac and the face-normal metric vectors
+    S1(1,:), S2(2,:) read once per stage."""
+    state = 13 * 72 / 5 if k_stage == 3 else 18 * 72 / 5
+    metrics = metric_bytes * (N * N + 4 * N * (N + 1)) / (N * N)
+    return state + metrics

@@ -47,6 +47,12 @@ struct sd2d_ctx {
   double resid_rms = 0.0, divb_rms = 0.0;
   int norms_valid = 0;
   bool have_av = false;
+  // profiling (sd2d_set_profiling)
+  bool prof = false;
+  std::vector<hipEvent_t> ev;     // pairs (start, stop)
+  size_t nev_used = 0;
+  double prof_ms = 0.0;
+  int prof_n = 0;
   std::string err;
 };
 
@@ -175,12 +181,41 @@ static hipError_t launch_stage_n(sd2d_ctx *c, const StageArgs &a, int mode) {
   return nr ? launch_stage_t<N, 2, M_RK | M_NORMS>(c, a) : launch_stage_t<N, 2, M_RK>(c, a);
 }
 
-static hipError_t launch_stage(sd2d_ctx *c, const StageArgs &a, int mode) {
+static hipError_t launch_stage_raw(sd2d_ctx *c, const StageArgs &a, int mode) {
   switch (c->N) {
     case 3: return launch_stage_n<3>(c, a, mode);
     case 4: return launch_stage_n<4>(c, a, mode);
     default: return launch_stage_n<5>(c, a, mode);
   }
+}
+
+static hipError_t launch_stage(sd2d_ctx *c, const StageArgs &a, int mode) {
+  if (!c->prof) return launch_stage_raw(c, a, mode);
+  while (c->ev.size() < c->nev_used + 2) {
+    hipEvent_t e;
+    hipError_t r = hipEventCreate(&e);
+    if (r != hipSuccess) return r;
+    c->ev.push_back(e);
+  }
+  hipError_t r = hipEventRecord(c->ev[c->nev_used], c->stream);
+  if (r != hipSuccess) return r;
+  r = launch_stage_raw(c, a, mode);
+  if (r != hipSuccess) return r;
+  r = hipEventRecord(c->ev[c->nev_used + 1], c->stream);
+  c->nev_used += 2;
+  return r;
+}
+
+// accumulate the recorded stage-kernel times (stream must be idle)
+static int collect_profile(sd2d_ctx *c) {
+  for (size_t i = 0; i + 1 < c->nev_used; i += 2) {
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev[i], c->ev[i + 1]));
+    c->prof_ms += ms;
+    c->prof_n += 1;
+  }
+  c->nev_used = 0;
+  return SD2D_OK;
 }
 
 static StageArgs base_args(sd2d_ctx *c) {
@@ -288,6 +323,7 @@ void sd2d_destroy(sd2d_ctx *c) {
   void *ptrs[] = {c->met, c->ftab, c->avtab, c->gmax, c->slots, c->sm_ind, c->normbuf,
                   c->normout, c->stagef, c->nablaf};
   for (void *b : ptrs) if (b) hipFree(b);
+  for (hipEvent_t e : c->ev) hipEventDestroy(e);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
 }
@@ -426,6 +462,11 @@ int sd2d_step(sd2d_ctx *c, int32_t nsteps, int32_t norm_every) {
     int r = step_rk(c, norms);
     if (r) return r;
     c->have_av = c->p.artiv == 1 && c->p.vismode == 1;
+    if (c->prof && c->nev_used >= 64) {   // bound the event pool
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+      r = collect_profile(c);
+      if (r) return r;
+    }
     if (norms) {
       hipLaunchKernelGGL(k_norm_reduce, dim3(1), dim3(256), 0, c->stream, c->normbuf, c->ntile,
                          c->normout);
@@ -440,6 +481,27 @@ int sd2d_step(sd2d_ctx *c, int32_t nsteps, int32_t norm_every) {
     }
   }
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (c->prof) return collect_profile(c);
+  return SD2D_OK;
+}
+
+int sd2d_set_profiling(sd2d_ctx *c, int32_t on) {
+  if (!c) return fail(nullptr, SD2D_EINVAL, "ctx is NULL");
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->prof = on != 0;
+  c->nev_used = 0;
+  c->prof_ms = 0.0;
+  c->prof_n = 0;
+  return SD2D_OK;
+}
+
+int sd2d_get_profile(sd2d_ctx *c, double *stage_ms, int32_t *nlaunch) {
+  if (!c) return fail(nullptr, SD2D_EINVAL, "ctx is NULL");
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  int r = collect_profile(c);
+  if (r) return r;
+  if (stage_ms) *stage_ms = c->prof_ms;
+  if (nlaunch) *nlaunch = c->prof_n;
   return SD2D_OK;
 }
 

@@ -23,7 +23,8 @@ _lib = None
 
 EXPORTS = ("sd2d_create", "sd2d_destroy", "sd2d_last_error", "sd2d_set_state",
            "sd2d_get_state", "sd2d_set_time", "sd2d_compresid",
-           "sd2d_compresid_q", "sd2d_step", "sd2d_get_diag", "sd2d_get_av")
+           "sd2d_compresid_q", "sd2d_step", "sd2d_get_diag", "sd2d_get_av",
+           "sd2d_set_profiling", "sd2d_get_profile")
 
 
 class Sd2dError(RuntimeError):
@@ -53,6 +54,8 @@ def load_library(path: str = LIBPATH):
     L.sd2d_step.argtypes = [vp, C.c_int32, C.c_int32]
     L.sd2d_get_diag.argtypes = [vp, C.POINTER(abi.Sd2dDiag)]
     L.sd2d_get_av.argtypes = [vp, vp, vp, C.POINTER(C.c_double)]
+    L.sd2d_set_profiling.argtypes = [vp, C.c_int32]
+    L.sd2d_get_profile.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int32)]
     for f in EXPORTS:
         if f not in ("sd2d_destroy", "sd2d_last_error"):
             getattr(L, f).restype = C.c_int
@@ -126,6 +129,15 @@ class Sd2d:
         d = abi.Sd2dDiag()
         self._check(self.L.sd2d_get_diag(self.h, C.byref(d)), "sd2d_get_diag")
         return {k: getattr(d, k) for k, _ in abi.Sd2dDiag._fields_}
+
+    def set_profiling(self, on: bool):
+        self._check(self.L.sd2d_set_profiling(self.h, int(bool(on))), "sd2d_set_profiling")
+
+    def get_profile(self):
+        """(summed stage-kernel milliseconds, number of stage launches)."""
+        ms = C.c_double(); n = C.c_int32()
+        self._check(self.L.sd2d_get_profile(self.h, C.byref(ms), C.byref(n)), "sd2d_get_profile")
+        return ms.value, n.value
 
     def get_av(self):
         s = np.zeros((2, self.nc), order="F"); m = np.zeros((2, self.nc), order="F")
