@@ -30,6 +30,24 @@ TIMOBJ := $(UNITS:%=$(OUT)/tim/%.o)
 
 all: $(OUT)/sd2d_MHD $(OUT)/sd2d_MHD_O3 $(OUT)/ref_driver
 
+# The reference program with its compresid replaced by the MI355X library
+# (sddc_amd/fortran): reference objects minus compresid.o, plus the
+# ISO_C_BINDING module and the drop-in compresid, linked to libsd2d.so.
+# Needs sddc_amd/lib/libsd2d.so built first.
+LIBDIR   := $(abspath $(HERE)../sddc_amd/lib)
+FDIR     := $(abspath $(HERE)../sddc_amd/fortran)
+DROPOBJ  := $(filter-out $(OUT)/fix/compresid.o,$(FIXOBJ))
+dropin: $(OUT)/sd2d_MHD_dropin
+
+$(OUT)/sd2d_MHD_dropin: $(OUT)/fix/.stamp $(FDIR)/sd2d_c.f90 $(FDIR)/compresid_sd2d.f90 $(LIBDIR)/libsd2d.so
+	mkdir -p $(OUT)/drop
+	$(FC) -c $(FIXFLAGS) -module-dir $(OUT)/drop $(FDIR)/sd2d_c.f90 -o $(OUT)/drop/sd2d_c.o
+	$(FC) -c $(FIXFLAGS) -I$(MPIINC) -I$(OUT)/fix -I$(OUT)/drop -module-dir $(OUT)/drop \
+	  $(FDIR)/compresid_sd2d.f90 -o $(OUT)/drop/compresid_sd2d.o
+	$(FC) $(FIXFLAGS) -o $@ $(OUT)/fix/main.o $(DROPOBJ) $(OUT)/drop/sd2d_c.o \
+	  $(OUT)/drop/compresid_sd2d.o -L$(LIBDIR) -lsd2d \
+	  -Wl,-rpath,'$$ORIGIN/../../sddc_amd/lib' -Wl,-rpath,/opt/rocm/lib $(MPILIB)
+
 $(OUT)/fix $(OUT)/tim:
 	mkdir -p $@
 
@@ -62,4 +80,4 @@ $(OUT)/ref_driver: $(OUT)/fix/.stamp $(HERE)ref_driver.f90
 clean:
 	rm -rf $(OUT)
 
-.PHONY: all clean
+.PHONY: all clean dropin

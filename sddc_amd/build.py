@@ -48,10 +48,12 @@ def build_oracle() -> str:
 
 def build_reference() -> bool:
     """Out-of-tree build of the reference Fortran (only where its sources
-    are present).  Returns False when there is nothing to build."""
+    are present), and of the reference program with its compresid replaced
+    by this library (needs the library built first).  Returns False when
+    there is nothing to build."""
     if not os.path.isdir("/root/reference"):
         return False
-    subprocess.run(["make", "-s", "-f", os.path.join(REPO, "oracle", "ref.mk")],
+    subprocess.run(["make", "-s", "-f", os.path.join(REPO, "oracle", "ref.mk"), "all", "dropin"],
                    check=True, cwd=REPO, stdout=subprocess.DEVNULL)
     return True
 

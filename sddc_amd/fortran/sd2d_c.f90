@@ -1,0 +1,174 @@
+! sd2d_c.f90 - ISO_C_BINDING interface to the C ABI of include/sd2d.h.
+!
+! The derived types mirror the C structs field by field (bind(C) gives
+! them the C layout, padding included); the functions are the C entry
+! points.  Array arguments are assumed-size dummies, so a caller passes
+! its module arrays (e.g. setup2d's Q, resid, nablaQs) directly, without
+! the TARGET attribute; optional dummies become NULL when absent.
+module sd2d_c
+  use iso_c_binding
+  implicit none
+  private
+
+  integer(c_int), parameter, public :: SD2D_OK = 0, SD2D_EINVAL = 1, &
+       SD2D_EHIP = 2, SD2D_ENOMEM = 3, SD2D_ENONFINITE = 4, &
+       SD2D_EUNSUPPORTED = 5
+  integer(c_int32_t), parameter, public :: SD2D_FP64 = 0, SD2D_MIXED = 1
+
+  type, bind(C), public :: sd2d_mesh_t
+     integer(c_int32_t) :: N, ncell, nface
+     type(c_ptr) :: if2c
+     integer(c_int32_t) :: ninter
+     type(c_ptr) :: ibfinter
+     integer(c_int32_t) :: ninlet
+     type(c_ptr) :: ibfinl
+     integer(c_int32_t) :: noutlet
+     type(c_ptr) :: ibfout
+     integer(c_int32_t) :: nwall
+     type(c_ptr) :: ibfwal
+     integer(c_int32_t) :: nsymp
+     type(c_ptr) :: ibfsymp
+     integer(c_int32_t) :: nfree
+     type(c_ptr) :: ibfree
+     integer(c_int32_t) :: npair
+     type(c_ptr) :: cyc2fpair
+     type(c_ptr) :: lmat, mmat, xs, xs_low, jac, s1, s2
+  end type sd2d_mesh_t
+
+  type, bind(C), public :: sd2d_params_t
+     integer(c_int32_t) :: vismode, artiv
+     real(c_double) :: mu, eta0, alpha
+     real(c_double) :: rinf, pinf, uinf, vinf, bxinf, byinf
+     real(c_double) :: dxcycl, dycycl
+     integer(c_int32_t) :: ncellglob, k_stage
+     real(c_double) :: dt
+     integer(c_int32_t) :: precision
+  end type sd2d_params_t
+
+  type, bind(C), public :: sd2d_maxima_t
+     real(c_double) :: eigvmax, umax, eigvmax_g, umax_g
+  end type sd2d_maxima_t
+
+  type, bind(C), public :: sd2d_diag_t
+     real(c_double) :: resid_rms, divb_rms, eigvmax_g, umax_g, c_h, epsilon0, ctime
+     integer(c_int32_t) :: iter, norms_valid
+  end type sd2d_diag_t
+
+  public :: sd2d_create, sd2d_destroy, sd2d_last_error, sd2d_set_state, &
+       sd2d_get_state, sd2d_set_time, sd2d_compresid, sd2d_compresid_q, &
+       sd2d_step, sd2d_get_diag, sd2d_get_av, sd2d_set_profiling, &
+       sd2d_get_profile, sd2d_error_message
+
+  interface
+     integer(c_int) function sd2d_create(mesh, params, ctx) bind(C, name="sd2d_create")
+       import :: c_int, c_ptr, sd2d_mesh_t, sd2d_params_t
+       type(sd2d_mesh_t), intent(in) :: mesh
+       type(sd2d_params_t), intent(in) :: params
+       type(c_ptr), intent(out) :: ctx
+     end function sd2d_create
+
+     subroutine sd2d_destroy(ctx) bind(C, name="sd2d_destroy")
+       import :: c_ptr
+       type(c_ptr), value :: ctx
+     end subroutine sd2d_destroy
+
+     type(c_ptr) function sd2d_last_error(ctx) bind(C, name="sd2d_last_error")
+       import :: c_ptr
+       type(c_ptr), value :: ctx
+     end function sd2d_last_error
+
+     integer(c_int) function sd2d_set_state(ctx, q, maxima) bind(C, name="sd2d_set_state")
+       import :: c_int, c_ptr, c_double, sd2d_maxima_t
+       type(c_ptr), value :: ctx
+       real(c_double), intent(in) :: q(*)
+       type(sd2d_maxima_t), intent(in), optional :: maxima
+     end function sd2d_set_state
+
+     integer(c_int) function sd2d_get_state(ctx, q, maxima) bind(C, name="sd2d_get_state")
+       import :: c_int, c_ptr, c_double, sd2d_maxima_t
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out), optional :: q(*)
+       type(sd2d_maxima_t), intent(out), optional :: maxima
+     end function sd2d_get_state
+
+     integer(c_int) function sd2d_set_time(ctx, ctime, iter) bind(C, name="sd2d_set_time")
+       import :: c_int, c_ptr, c_double, c_int32_t
+       type(c_ptr), value :: ctx
+       real(c_double), value :: ctime
+       integer(c_int32_t), value :: iter
+     end function sd2d_set_time
+
+     integer(c_int) function sd2d_compresid(ctx, resid, nablaqs) bind(C, name="sd2d_compresid")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out), optional :: resid(*)
+       real(c_double), intent(out), optional :: nablaqs(*)
+     end function sd2d_compresid
+
+     integer(c_int) function sd2d_compresid_q(ctx, q, resid, nablaqs) &
+          bind(C, name="sd2d_compresid_q")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(in) :: q(*)
+       real(c_double), intent(out), optional :: resid(*)
+       real(c_double), intent(out), optional :: nablaqs(*)
+     end function sd2d_compresid_q
+
+     integer(c_int) function sd2d_step(ctx, nsteps, norm_every) bind(C, name="sd2d_step")
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: nsteps, norm_every
+     end function sd2d_step
+
+     integer(c_int) function sd2d_get_diag(ctx, diag) bind(C, name="sd2d_get_diag")
+       import :: c_int, c_ptr, sd2d_diag_t
+       type(c_ptr), value :: ctx
+       type(sd2d_diag_t), intent(out) :: diag
+     end function sd2d_get_diag
+
+     integer(c_int) function sd2d_get_av(ctx, sm_ind, muav, epsilon0) bind(C, name="sd2d_get_av")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out), optional :: sm_ind(*)
+       real(c_double), intent(out), optional :: muav(*)
+       real(c_double), intent(out), optional :: epsilon0
+     end function sd2d_get_av
+
+     integer(c_int) function sd2d_set_profiling(ctx, on) bind(C, name="sd2d_set_profiling")
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: on
+     end function sd2d_set_profiling
+
+     integer(c_int) function sd2d_get_profile(ctx, stage_ms, nlaunch) &
+          bind(C, name="sd2d_get_profile")
+       import :: c_int, c_ptr, c_double, c_int32_t
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out), optional :: stage_ms
+       integer(c_int32_t), intent(out), optional :: nlaunch
+     end function sd2d_get_profile
+  end interface
+
+contains
+
+  ! The message of sd2d_last_error as a Fortran string.
+  function sd2d_error_message(ctx) result(msg)
+    type(c_ptr), intent(in) :: ctx
+    character(len=:), allocatable :: msg
+    character(kind=c_char), pointer :: s(:)
+    integer :: n
+    interface
+       integer(c_size_t) function c_strlen(p) bind(C, name="strlen")
+         import :: c_size_t, c_ptr
+         type(c_ptr), value :: p
+       end function c_strlen
+    end interface
+    type(c_ptr) :: p
+    p = sd2d_last_error(ctx)
+    n = int(c_strlen(p))
+    call c_f_pointer(p, s, [n])
+    allocate(character(len=n) :: msg)
+    msg = transfer(s(1:n), msg)
+  end function sd2d_error_message
+
+end module sd2d_c

@@ -590,11 +590,17 @@ def conserved(rho, u, v, w, p, bx, by, bz, psi, gam=GAM):
 
 
 def freestream_state(case: Case, rho_pert: float = 0.0, uv_pert: float = 0.0,
-                     seed: int = 0) -> np.ndarray:
+                     seed: int = 0, wall_ramp: tuple[float, float] | None = None
+                     ) -> np.ndarray:
     """Free stream of the run parameters, optionally with a seeded uniform
     density perturbation rho*(1+rho_pert*U[-1,1]) at fixed pressure and a
     seeded 0..uv_pert perturbation of u and v (the reference's own
-    perturbation, setinitialcond.f90:17-18, with a seed)."""
+    perturbation, setinitialcond.f90:17-18, with a seed).
+
+    wall_ramp=(r0, delta) starts the flow at rest on the circle r=r0 and
+    lets the velocity rise smoothly to the free stream,
+    u = u_inf*(1 - exp(-((r - r0)/delta)^2)), instead of the impulsive
+    start of a uniform free stream against the wall."""
     p = case.params; N = p.N; nc = case.ncell
     rng = np.random.default_rng(seed)
     shp = (N, N, nc)
@@ -602,6 +608,11 @@ def freestream_state(case: Case, rho_pert: float = 0.0, uv_pert: float = 0.0,
     if rho_pert:
         rho = rho * (1.0 + rho_pert * rng.uniform(-1.0, 1.0, size=shp))
     u = np.full(shp, p.uinf); v = np.full(shp, p.vinf)
+    if wall_ramp is not None:
+        xy = solution_coords(case.cells, case.mesh.xv, case.mesh.yv, N)
+        r = np.hypot(xy[0], xy[1])
+        s = 1.0 - np.exp(-(np.maximum(r - wall_ramp[0], 0.0) / wall_ramp[1]) ** 2)
+        u = u * s; v = v * s
     if uv_pert:
         u = u + uv_pert * rng.random(shp)
         v = v + uv_pert * rng.random(shp)

@@ -15,7 +15,7 @@ produced:
   final_*    state and maxima after whole RK_S5P3 / FiveStageRk steps
   prog_*     the unmodified program's resid.out rows and restart state
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [name ...]     # default: all fixtures
 """
 from __future__ import annotations
 
@@ -48,9 +48,11 @@ def shrink(key: str, v: np.ndarray, big: bool, out: dict) -> None:
     out[key + "__linf"] = np.abs(w).max(axis=1)
 
 
-def main() -> None:
+def main(names: list[str]) -> None:
     refrun.build_reference()
     for name, (mspec, pspec, ispec, nstaged, nrk) in SPECS.items():
+        if names and name not in names:
+            continue
         case, Q0 = build(name)
         res = refrun.run_driver(case, Q0, nstaged=nstaged, nrk=nrk)
         out = {"meta": json.dumps({"mesh": mspec, "params": pspec, "state": ispec,
@@ -84,4 +86,4 @@ def main() -> None:
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])

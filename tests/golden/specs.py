@@ -28,7 +28,7 @@ SPECS = {
                          dict(N=4, dt=1e-3, k_stage=5), dict(kind="smooth", amp=0.05, seed=4), 0, 3),
     "c1_cylinder": (dict(kind="ogrid", nr=32, nt=64, r0=0.5, r1=15.0),
                     dict(N=4, dt=5e-4),
-                    dict(kind="freestream", uv_pert=0.02, seed=5), 1, 2),
+                    dict(kind="freestream", uv_pert=0.02, seed=5, ramp=0.5), 1, 2),
 }
 
 
@@ -50,8 +50,10 @@ def make_state(case: M.Case, spec: dict) -> np.ndarray:
     if spec["kind"] == "smooth":
         return M.smooth_perturbed_state(case, spec["amp"], seed=spec["seed"])
     if spec["kind"] == "freestream":
+        ramp = spec.get("ramp")
         return M.freestream_state(case, uv_pert=spec.get("uv_pert", 0.0),
-                                  rho_pert=spec.get("rho_pert", 0.0), seed=spec["seed"])
+                                  rho_pert=spec.get("rho_pert", 0.0), seed=spec["seed"],
+                                  wall_ramp=None if ramp is None else (0.5, ramp))
     raise ValueError(spec["kind"])
 
 
