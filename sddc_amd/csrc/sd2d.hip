@@ -2,9 +2,10 @@
 // RK path: context set-up, device layouts, and the launch sequence.
 //
 // Device layout ("tile-major SoA"): cells are grouped in tiles of CPW
-// cells (the cells of one wave); inside a tile every (variable, point)
-// pair is a contiguous run of CPW doubles.  A wave therefore reads and
-// writes whole runs, and a tile of state is one contiguous block.
+// cells (the cells of one workgroup of the stage kernel); inside a tile
+// every (variable, point) pair is a contiguous run of CPW doubles, so the
+// threads of one grid line read and write whole runs, and a tile of state
+// is one contiguous block.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -158,12 +159,12 @@ __global__ void k_norm_reduce(const double *__restrict__ nb, int ntile,
 // ---------------------------------------------------------------------------
 // stage launch
 // ---------------------------------------------------------------------------
-template <int N, int GV, int MODE>
+template <int N, int MODE>
 static hipError_t launch_stage_t(sd2d_ctx *c, const StageArgs &a) {
-  constexpr size_t lds = stage_lds_bytes<N, GV>();
+  constexpr size_t lds = stage_lds_bytes<N, MODE>();
   static_assert(lds <= 65536, "stage kernel LDS above 64 KiB");
-  const int grid = c->ntile * Lds<N, GV>::NSUB;
-  hipLaunchKernelGGL((stage_kernel<N, GV, MODE>), dim3(grid), dim3(STAGE_THREADS), lds,
+  static_assert(2 * N * Geo<N>::CPW <= STAGE_THREADS, "one thread per grid line");
+  hipLaunchKernelGGL((stage_kernel<N, MODE>), dim3(c->ntile), dim3(STAGE_THREADS), lds,
                      c->stream, a, c->ops, c->ph);
   return hipGetLastError();
 }
@@ -172,13 +173,12 @@ template <int N>
 static hipError_t launch_stage_n(sd2d_ctx *c, const StageArgs &a, int mode) {
   const bool av = (mode & M_AV) != 0;
   if (mode & M_COMPAT) {
-    return av ? launch_stage_t<N, 9, M_COMPAT | M_AV>(c, a)
-              : launch_stage_t<N, 9, M_COMPAT>(c, a);
+    return av ? launch_stage_t<N, M_COMPAT | M_AV>(c, a) : launch_stage_t<N, M_COMPAT>(c, a);
   }
   const bool nr = (mode & M_NORMS) != 0;
-  if (av) return nr ? launch_stage_t<N, 2, M_RK | M_AV | M_NORMS>(c, a)
-                    : launch_stage_t<N, 2, M_RK | M_AV>(c, a);
-  return nr ? launch_stage_t<N, 2, M_RK | M_NORMS>(c, a) : launch_stage_t<N, 2, M_RK>(c, a);
+  if (av) return nr ? launch_stage_t<N, M_RK | M_AV | M_NORMS>(c, a)
+                    : launch_stage_t<N, M_RK | M_AV>(c, a);
+  return nr ? launch_stage_t<N, M_RK | M_NORMS>(c, a) : launch_stage_t<N, M_RK>(c, a);
 }
 
 static hipError_t launch_stage_raw(sd2d_ctx *c, const StageArgs &a, int mode) {

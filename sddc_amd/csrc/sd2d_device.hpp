@@ -15,9 +15,11 @@ namespace sd2d {
 constexpr int NMAX = 5;
 constexpr int NSLOT = 256;          // running-maximum slots (spread atomics)
 
-// Cells per tile (one workgroup of the stage kernel) for each order.
+// Cells per tile for each order.  One 128-thread workgroup of the stage
+// kernel owns one tile and runs one thread per grid line of each cell (2N
+// lines per cell): 120, 128 and 126 busy threads for N = 5, 4, 3.
 __host__ __device__ constexpr int cpw_of(int n) {
-  return n == 5 ? 8 : (n == 4 ? 12 : 20);
+  return n == 5 ? 12 : (n == 4 ? 16 : 21);
 }
 
 template <int N> struct Geo {
@@ -59,10 +61,13 @@ __host__ __device__ inline int finfo_pack(int kn, int left, int bc, int sneg) {
 constexpr double GAM = 5.0 / 3;   // setup2d.f90:67
 
 // ---------------------------------------------------------------------------
-// getIfluxvectors (getrusanov.f90:127-193): ideal-MHD + GLM flux vectors.
-// Only the running maximum of |u|,|v|,|w| is returned through um.
+// getIfluxvectors (getrusanov.f90:127-193) contracted with the metric
+// vector (sx, sy) of the flux point, as inviscid_flux_interior stores it
+// (F1 = F*S1(1,1) + G*S1(1,2), compflux.f90:79,97): the ideal-MHD + GLM
+// flux normal to (sx, sy), formed directly from the normal velocity and
+// normal field.  The running maximum of |u|,|v|,|w| is returned through um.
 template <typename T>
-__device__ __forceinline__ void iflux(const T *q, T c_h, T *F, T *G, T &um) {
+__device__ __forceinline__ void iflux_n(const T *q, T c_h, T sx, T sy, T *Fn, T &um) {
   const T rho = q[0], rhou = q[1], rhov = q[2], rhow = q[3], rhoE = q[4];
   const T Bx = q[5], By = q[6], Bz = q[7], psi = q[8];
   const T ri = T(1) / rho;
@@ -71,26 +76,21 @@ __device__ __forceinline__ void iflux(const T *q, T c_h, T *F, T *G, T &um) {
   const T pr = T(GAM - 1.0) *
                (rhoE - T(0.5) * (rhou * u + rhov * v + rhow * w) - T(0.5) * B2 -
                 T(0.5) * psi * psi);
+  const T pt = pr + T(0.5) * B2;
   const T ub = u * Bx + v * By + w * Bz;
-  const T ef = rhoE - T(0.5) * psi * psi + pr + T(0.5) * B2;
-  F[0] = rhou;
-  F[1] = rhou * u + pr + (T(0.5) * B2 - Bx * Bx);
-  F[2] = rhou * v - Bx * By;
-  F[3] = rhou * w - Bx * Bz;
-  F[4] = u * ef - ub * Bx + c_h * psi * Bx;
-  F[5] = c_h * psi;
-  F[6] = u * By - v * Bx;
-  F[7] = u * Bz - w * Bx;
-  F[8] = c_h * Bx;
-  G[0] = rhov;
-  G[1] = rhou * v - Bx * By;
-  G[2] = rhov * v + pr + (T(0.5) * B2 - By * By);
-  G[3] = rhow * v - By * Bz;
-  G[4] = v * ef - ub * By + c_h * psi * By;
-  G[5] = v * Bx - u * By;
-  G[6] = c_h * psi;
-  G[7] = v * Bz - w * By;
-  G[8] = c_h * By;
+  const T ef = rhoE - T(0.5) * psi * psi + pt;
+  const T un = u * sx + v * sy;
+  const T bn = Bx * sx + By * sy;
+  const T cp = c_h * psi;
+  Fn[0] = rhou * sx + rhov * sy;
+  Fn[1] = rhou * un + pt * sx - Bx * bn;
+  Fn[2] = rhov * un + pt * sy - By * bn;
+  Fn[3] = rhow * un - Bz * bn;
+  Fn[4] = un * ef - ub * bn + cp * bn;
+  Fn[5] = cp * sx + (v * Bx - u * By) * sy;
+  Fn[6] = (u * By - v * Bx) * sx + cp * sy;
+  Fn[7] = Bz * un - w * bn;
+  Fn[8] = c_h * bn;
   T ut = fmax(fmax(fabs(u), fabs(v)), fabs(w));
   if (ut > um) um = ut;
 }

@@ -41,7 +41,10 @@ int s5p3_sequence(RkBuffers &rb, double dt, bool norms, int av, const StageArgs 
     a.naux = st[s].aux ? 1 : 0;
     a.aux[0] = st[s].aux;
     a.ca[0] = st[s].ca;
-    const int mode = M_RK | av | ((norms && s == 4) ? M_NORMS : 0);
+    // the AV sensor is a diagnostic (it does not feed back), so it is
+    // formed on the last stage only: after every step SM_IND is the last
+    // residual evaluation's, as in the reference
+    const int mode = M_RK | (s == 4 ? av : 0) | ((norms && s == 4) ? M_NORMS : 0);
     const int r = launch(a, mode);
     if (r) return r;
   }
@@ -63,7 +66,7 @@ int fsrk_sequence(RkBuffers &rb, double dt, bool norms, int av, const StageArgs 
   double *qj2 = rb.fsr[0], *qj3 = rb.fsr[1], *res0 = rb.fsr[2];
   for (int s = 0; s < 5; ++s) {
     StageArgs a = base;
-    int mode = M_RK | av;
+    int mode = M_RK | (s == 4 ? av : 0);   // AV sensor on the last stage only
     switch (s) {
       case 0: a.qin = Q0; a.qout = X; a.cin = 1.0; a.cr = dt * a11; break;
       case 1: a.qin = X; a.qout = qj2; a.cin = a22; a.cr = a23 * dt;

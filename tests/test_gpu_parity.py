@@ -61,3 +61,72 @@ def test_rk_steps_parity(name, k_stage):
     d = per_var_error(Qg - Q0, Qo - Q0)[0]
     assert d.max() <= 1e-11, (name, d)
     assert np.allclose(mg, mo, rtol=1e-13, atol=0), (mg, mo)
+
+
+# The AV smoothness sensor divides a squared small difference by a squared
+# value (AV.f90:65-81), so rounding differences of 1e-15 in its inputs move
+# log10(Se) by up to ~1e-8; it is a diagnostic with no effect on the flow.
+SM_TOL = 1e-6
+
+
+@pytest.mark.parametrize("name", ORDERED)
+def test_av_sensor_parity(name):
+    """SM_IND, muAV and epsilon0 after a compresid call and after whole
+    steps (the sensor of the last stage) match the oracle's."""
+    print(f"[gpu-parity] AV sensor {name}", flush=True)
+    case, Q0 = make_case(name)
+    g = _gpu()(case)
+    o = _oracle()(case)
+    g.set_state(Q0, [0, 0, 0, 0]); o.set_state(Q0, [0, 0, 0, 0])
+    for what in ("compresid", "steps"):
+        if what == "compresid":
+            g.compresid(); o.compresid()
+        else:
+            g.step(2); o.step(2)
+        sg, mg, eg = g.get_av()
+        so, mo = o.get_av()
+        so = np.asarray(so); mo = np.asarray(mo)
+        fin = np.isfinite(so)
+        assert np.array_equal(np.isfinite(sg), fin), (name, what)
+        assert np.abs(sg[fin] - so[fin]).max() <= SM_TOL, (name, what)
+        np.testing.assert_allclose(mg, mo, rtol=1e-6, atol=1e-12 * max(np.abs(mo).max(), 1e-300))
+        np.testing.assert_allclose(eg, o.diag()["epsilon0"], rtol=1e-14)
+
+
+@pytest.mark.parametrize("name", ["ogrid4x8_N4", "channel7x6_N5"])
+def test_norms_parity(name):
+    """Residual and divB RMS norms of iterations.f90:34-62 from sd2d_step
+    (steps with mod(iter,10) == 1) match the oracle's."""
+    print(f"[gpu-parity] norms {name}", flush=True)
+    case, Q0 = make_case(name)
+    g = _gpu()(case)
+    o = _oracle()(case)
+    g.set_state(Q0, [0, 0, 0, 0]); o.set_state(Q0, [0, 0, 0, 0])
+    g.step(11, 10); o.step(11, 10)
+    dg, do = g.diag(), o.diag()
+    assert dg["norms_valid"] == 1 and dg["iter"] == 11
+    np.testing.assert_allclose([dg["resid_rms"], dg["divb_rms"]],
+                               [do["resid_rms"], do["divb_rms"]], rtol=1e-10)
+    t_o, it_o = o.get_time()
+    assert it_o == 11
+    np.testing.assert_allclose(dg["ctime"], t_o, rtol=1e-15)
+
+
+def test_determinism():
+    """Two runs of the same steps give bitwise the same state, maxima and
+    norms (maxima by atomicMax on bit patterns, norms summed in a fixed
+    order)."""
+    print("[gpu-parity] determinism", flush=True)
+    case, Q0 = make_case("channel7x6_N5")
+    res = []
+    for _ in range(2):
+        g = _gpu()(case)
+        g.set_state(Q0, [0, 0, 0, 0])
+        g.step(3, 2)
+        Q, mx = g.get_state()
+        d = g.diag()
+        res.append((Q, mx, d["resid_rms"], d["divb_rms"]))
+        g.close()
+    assert np.array_equal(res[0][0], res[1][0])
+    assert np.array_equal(res[0][1], res[1][1])
+    assert res[0][2:] == res[1][2:]
