@@ -35,6 +35,7 @@ struct sd2d_ctx {
   double *fsr[4] = {nullptr, nullptr, nullptr, nullptr};  // FiveStageRk registers
   double *met = nullptr;
   int *ftab = nullptr;
+  int *torder = nullptr;
   double *avtab = nullptr;
   double *gmax = nullptr;                       // eigvmax_g, umax_g
   unsigned long long *slots = nullptr;
@@ -169,16 +170,27 @@ static hipError_t launch_stage_t(sd2d_ctx *c, const StageArgs &a) {
   return hipGetLastError();
 }
 
+// the stage kernels that the RK sequences and sd2d_compresid launch
 template <int N>
 static hipError_t launch_stage_n(sd2d_ctx *c, const StageArgs &a, int mode) {
-  const bool av = (mode & M_AV) != 0;
-  if (mode & M_COMPAT) {
-    return av ? launch_stage_t<N, M_COMPAT | M_AV>(c, a) : launch_stage_t<N, M_COMPAT>(c, a);
+  constexpr int R = M_RK, A1 = M_AUX1, A4 = M_AUX4, V = M_AV, NR = M_NORMS;
+  switch (mode) {
+#define SD2D_MODE(m) case (m): return launch_stage_t<N, (m)>(c, a);
+    SD2D_MODE(R)
+    SD2D_MODE(R | A1)
+    SD2D_MODE(R | A1 | V)
+    SD2D_MODE(R | A1 | NR)
+    SD2D_MODE(R | A1 | V | NR)
+    SD2D_MODE(R | A1 | M_RSTORE)
+    SD2D_MODE(R | A4)
+    SD2D_MODE(R | A4 | V)
+    SD2D_MODE(R | A4 | NR)
+    SD2D_MODE(R | A4 | V | NR)
+    SD2D_MODE(M_COMPAT)
+    SD2D_MODE(M_COMPAT | V)
+#undef SD2D_MODE
+    default: return hipErrorInvalidValue;
   }
-  const bool nr = (mode & M_NORMS) != 0;
-  if (av) return nr ? launch_stage_t<N, M_RK | M_AV | M_NORMS>(c, a)
-                    : launch_stage_t<N, M_RK | M_AV>(c, a);
-  return nr ? launch_stage_t<N, M_RK | M_NORMS>(c, a) : launch_stage_t<N, M_RK>(c, a);
 }
 
 static hipError_t launch_stage_raw(sd2d_ctx *c, const StageArgs &a, int mode) {
@@ -222,6 +234,7 @@ static StageArgs base_args(sd2d_ctx *c) {
   StageArgs a{};
   a.met = c->met;
   a.ftab = c->ftab;
+  a.torder = c->torder;
   a.gmax = c->gmax;
   a.slots = c->slots;
   a.avtab = c->avtab;
@@ -294,6 +307,7 @@ int sd2d_create(const sd2d_mesh *m, const sd2d_params *p, sd2d_ctx **out) {
     for (int i = 0; i < 4; ++i) CK(hipMalloc(&c->fsr[i], qn * sizeof(double)));
   CK(hipMalloc(&c->met, met.size() * sizeof(double)));
   CK(hipMalloc(&c->ftab, ft.size() * sizeof(int)));
+  CK(hipMalloc(&c->torder, tb.torder.size() * sizeof(int)));
   CK(hipMalloc(&c->avtab, av.size() * sizeof(double)));
   CK(hipMalloc(&c->gmax, 2 * sizeof(double)));
   CK(hipMalloc(&c->slots, 2 * NSLOT * sizeof(unsigned long long)));
@@ -303,6 +317,8 @@ int sd2d_create(const sd2d_mesh *m, const sd2d_params *p, sd2d_ctx **out) {
   CK(hipMalloc(&c->stagef, (size_t)nc * 9 * N2 * sizeof(double)));
   CK(hipMemcpyAsync(c->met, met.data(), met.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
   CK(hipMemcpyAsync(c->ftab, ft.data(), ft.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  CK(hipMemcpyAsync(c->torder, tb.torder.data(), tb.torder.size() * sizeof(int),
+                    hipMemcpyHostToDevice, c->stream));
   CK(hipMemcpyAsync(c->avtab, av.data(), av.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
   for (int i = 0; i < 3; ++i) CK(hipMemsetAsync(c->q[i], 0, qn * sizeof(double), c->stream));
   CK(hipMemsetAsync(c->sm_ind, 0, 2 * (size_t)nc * sizeof(double), c->stream));
@@ -320,7 +336,7 @@ void sd2d_destroy(sd2d_ctx *c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   for (double *b : c->q) if (b) hipFree(b);
   for (double *b : c->fsr) if (b) hipFree(b);
-  void *ptrs[] = {c->met, c->ftab, c->avtab, c->gmax, c->slots, c->sm_ind, c->normbuf,
+  void *ptrs[] = {c->met, c->ftab, c->torder, c->avtab, c->gmax, c->slots, c->sm_ind, c->normbuf,
                   c->normout, c->stagef, c->nablaf};
   for (void *b : ptrs) if (b) hipFree(b);
   for (hipEvent_t e : c->ev) hipEventDestroy(e);

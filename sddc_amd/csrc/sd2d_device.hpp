@@ -17,9 +17,10 @@ constexpr int NSLOT = 256;          // running-maximum slots (spread atomics)
 
 // Cells per tile for each order.  One 128-thread workgroup of the stage
 // kernel owns one tile and runs one thread per grid line of each cell (2N
-// lines per cell): 120, 128 and 126 busy threads for N = 5, 4, 3.
+// lines per cell): 120, 128 and 120 busy threads for N = 5, 4, 3.  Every
+// tile of state is a multiple of 16 bytes.
 __host__ __device__ constexpr int cpw_of(int n) {
-  return n == 5 ? 12 : (n == 4 ? 16 : 21);
+  return n == 5 ? 12 : (n == 4 ? 16 : 20);
 }
 
 template <int N> struct Geo {

@@ -2,8 +2,10 @@
 // the per-cell face table, the tile-major metrics and the AV projections,
 // built once from the arrays module setup2d holds after INIT_SETUP.
 #pragma once
+#include <algorithm>
 #include <cstdint>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/sd2d.h"
@@ -15,6 +17,7 @@ struct HostTables {
   int N = 0, ncell = 0, cpw = 0, ntile = 0;
   Ops ops{};
   Phys ph{};
+  std::vector<int> torder;   // launch order of the tiles (see tile_order)
   std::vector<int> ft;       // [tile][k][{neighbour base, info}][ci]
   std::vector<double> met;   // [tile][metric slot][ci]
   std::vector<double> av;    // hlow[isl][is], then hhigh[is][isl]
@@ -142,6 +145,77 @@ inline int build_face_table(const sd2d_mesh *m, int CPW, std::vector<int> &nb,
   return SD2D_OK;
 }
 
+// Launch order of the tiles: recursive bisection of the tile adjacency
+// graph (two tiles are adjacent when a face joins their cells).  Each part
+// is split at the median of the breadth-first distance from a
+// pseudo-peripheral tile of the part, so that every part, down to groups
+// of a few tiles, is a compact patch of the mesh and consecutive tiles in
+// the order are neighbours.  Tiles that are processed close together in
+// time then find each other's states in the L2 cache of their XCD,
+// whatever the cell numbering of the mesh.
+inline std::vector<int> tile_order(int nt, int CPW, int tile_doubles,
+                                   const std::vector<int> &nb, const std::vector<int> &info) {
+  // adjacency (CSR, deduplicated)
+  std::vector<std::vector<int>> adj(nt);
+  const int nc = (int)(nb.size() / 4);
+  for (int cell = 0; cell < nc; ++cell)
+    for (int k = 0; k < 4; ++k) {
+      const size_t i = 4 * (size_t)cell + k;
+      if (((info[i] >> 3) & 7) != FT_INTERIOR) continue;
+      const int t = cell / CPW, u = nb[i] / tile_doubles;
+      if (u != t) adj[t].push_back(u);
+    }
+  std::vector<int> xadj(nt + 1, 0), ladj;
+  for (int t = 0; t < nt; ++t) {
+    std::sort(adj[t].begin(), adj[t].end());
+    adj[t].erase(std::unique(adj[t].begin(), adj[t].end()), adj[t].end());
+    xadj[t + 1] = xadj[t] + (int)adj[t].size();
+    ladj.insert(ladj.end(), adj[t].begin(), adj[t].end());
+  }
+  std::vector<int> ord(nt), part(nt, -1), dist(nt, -1), queue;
+  for (int t = 0; t < nt; ++t) ord[t] = t;
+  queue.reserve(nt);
+  // breadth-first distances inside part `pid` from `src`; returns the last
+  // tile reached (a farthest one)
+  auto bfs = [&](int pid, int src, int b, int e) {
+    for (int i = b; i < e; ++i) dist[ord[i]] = -1;
+    queue.clear();
+    queue.push_back(src);
+    dist[src] = 0;
+    for (size_t h = 0; h < queue.size(); ++h) {
+      const int t = queue[h];
+      for (int j = xadj[t]; j < xadj[t + 1]; ++j) {
+        const int u = ladj[j];
+        if (part[u] == pid && dist[u] < 0) { dist[u] = dist[t] + 1; queue.push_back(u); }
+      }
+    }
+    return queue.back();
+  };
+  constexpr int LEAF = 4;
+  std::vector<std::pair<int, int>> stack{{0, nt}};
+  int pid = 0;
+  while (!stack.empty()) {
+    const auto [b, e] = stack.back();
+    stack.pop_back();
+    if (e - b <= LEAF) continue;
+    ++pid;
+    for (int i = b; i < e; ++i) part[ord[i]] = pid;
+    // pseudo-peripheral start: the farthest tile from the lowest-numbered one
+    const int first = *std::min_element(ord.begin() + b, ord.begin() + e);
+    const int p = bfs(pid, first, b, e);
+    bfs(pid, p, b, e);
+    std::stable_sort(ord.begin() + b, ord.begin() + e, [&](int x, int y) {
+      const long long dx = dist[x] < 0 ? (1LL << 40) : dist[x];
+      const long long dy = dist[y] < 0 ? (1LL << 40) : dist[y];
+      return dx != dy ? dx < dy : x < y;
+    });
+    const int m = b + (e - b) / 2;
+    stack.push_back({m, e});     // second half after the first
+    stack.push_back({b, m});
+  }
+  return ord;
+}
+
 // Everything the kernels read besides the state.
 inline int build_host_tables(const sd2d_mesh *m, const sd2d_params *p, HostTables &t,
                              std::string &err) {
@@ -161,6 +235,7 @@ inline int build_host_tables(const sd2d_mesh *m, const sd2d_params *p, HostTable
   const int r = build_face_table(m, CPW, nb, info, err);
   if (r) return r;
   const int nt = t.ntile;
+  t.torder = tile_order(nt, CPW, 9 * N2 * CPW, nb, info);
   t.ft.assign((size_t)nt * 8 * CPW, 0);
   for (int cell = 0; cell < nc; ++cell)
     for (int k = 0; k < 4; ++k) {

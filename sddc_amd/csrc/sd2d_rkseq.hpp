@@ -44,7 +44,8 @@ int s5p3_sequence(RkBuffers &rb, double dt, bool norms, int av, const StageArgs 
     // the AV sensor is a diagnostic (it does not feed back), so it is
     // formed on the last stage only: after every step SM_IND is the last
     // residual evaluation's, as in the reference
-    const int mode = M_RK | (s == 4 ? av : 0) | ((norms && s == 4) ? M_NORMS : 0);
+    const int mode = M_RK | (a.naux ? M_AUX1 : 0) | (s == 4 ? av : 0) |
+                     ((norms && s == 4) ? M_NORMS : 0);
     const int r = launch(a, mode);
     if (r) return r;
   }
@@ -85,6 +86,8 @@ int fsrk_sequence(RkBuffers &rb, double dt, bool norms, int av, const StageArgs 
         if (norms) mode |= M_NORMS;
         break;
     }
+    mode |= a.naux == 4 ? M_AUX4 : (a.naux == 1 ? M_AUX1 : 0);
+    if (a.rstore) mode |= M_RSTORE;
     const int r = launch(a, mode);
     if (r) return r;
   }

@@ -13,7 +13,7 @@
 //     component of the averaged field (divB) and its rotated counterpart
 //     (Jz, for the AV sensor) likewise.
 // The eta-line threads (fam 1) hand their derivatives to the xi-line
-// threads (fam 0) through LDS, once; each xi-line thread then finishes the
+// threads (fam 0) through LDS, in two rounds; each xi-line thread then finishes the
 // N solution points of its line: residual, Powell and GLM sources, the RK
 // combination, norm partials.  The running maxima and norms are reduced
 // per workgroup; the AV smoothness sensor runs per cell on the last stage.
@@ -45,6 +45,7 @@ struct StageArgs {
   double *rstore;           // resid (tile layout) for FiveStageRk, or null
   const double *met;        // metrics (tile layout)
   const int *ftab;          // per-cell face table (tile layout)
+  const int *torder;        // launch order of the tiles
   const double *gmax;       // eigvmax_g, umax_g of the previous evaluation
   unsigned long long *slots;  // running maxima, NSLOT x 2 (bit patterns)
   double *sm_ind;           // SM_IND(2,NCELL) or null
@@ -55,12 +56,15 @@ struct StageArgs {
   int ncell, ntile;
 };
 
-// MODE bits
-enum { M_RK = 1, M_AV = 2, M_NORMS = 4, M_COMPAT = 8 };
+// MODE bits.  The RK registers read pointwise (none, one, or the four of
+// FiveStageRk's last stage) and the residual store are part of the mode so
+// that the stage update is straight-line code.
+enum { M_RK = 1, M_AV = 2, M_NORMS = 4, M_COMPAT = 8, M_AUX1 = 16, M_AUX4 = 32,
+       M_RSTORE = 64 };
 
-// XCD-aware tile order: consecutive tiles go to the same XCD (and its L2)
-// so neighbour lines read by nearby tiles hit in cache.  Bijective for
-// any tile count.
+// XCD-aware block order: consecutive positions of the tile launch order go
+// to the same XCD (and its L2), so the neighbour lines a tile reads were
+// staged by a nearby tile on the same XCD.  Bijective for any tile count.
 __device__ __forceinline__ int tile_of_block(int b, int nt) {
   const int q = nt >> 3, r = nt & 7;
   const int x = b & 7, i = b >> 3;
@@ -68,25 +72,27 @@ __device__ __forceinline__ int tile_of_block(int b, int nt) {
 }
 
 // LDS of one workgroup (in doubles): the tile's state [v][p][ci], the
-// eta-line derivatives of the fluxes [v][p][ci], of the divB weight
-// [p][ci] and (AV) of the Jz weight [p][ci], then reduction slots.  The AV
-// sensor reuses the derivative area once the residual is formed.
+// eta-line derivatives of the fluxes of XV variables at a time [v][p][ci]
+// (two rounds cover the nine), of the divB weight [p][ci] and (AV) of the
+// Jz weight [p][ci], then reduction slots.  The AV sensor reuses the state
+// area once the residual is formed.
 template <int N, int MODE> struct StageLds {
   static constexpr int T = Geo<N>::CPW, N2 = N * N, NL = N - 1;
   static constexpr bool AV = (MODE & M_AV) != 0;
+  static constexpr int XV = 5;                 // variables per exchange round
   static constexpr int Q0 = 0;
   static constexpr int XD = 9 * N2 * T;
-  static constexpr int XB = XD + 9 * N2 * T;
+  static constexpr int XB = XD + XV * N2 * T;
   static constexpr int XJ = XB + N2 * T;
   static constexpr int RED = XJ + (AV ? N2 * T : 0);
   static constexpr int TOTAL = RED + 4 * (STAGE_THREADS / 64);
-  // AV sensor scratch inside [XD, XB): rho, Jz^2 [p][ci], low-order
+  // AV sensor scratch inside [Q0, XD): rho, Jz^2 [p][ci], low-order
   // projections [2][NL*NL][ci], point errors [2][N2][ci]
-  static constexpr int A_RHO = XD;
-  static constexpr int A_JZ = XD + N2 * T;
-  static constexpr int A_LOW = XD + 2 * N2 * T;
+  static constexpr int A_RHO = Q0;
+  static constexpr int A_JZ = Q0 + N2 * T;
+  static constexpr int A_LOW = Q0 + 2 * N2 * T;
   static constexpr int A_SE = A_LOW + 2 * NL * NL * T;
-  static_assert(A_SE + 2 * N2 * T <= XB, "AV scratch exceeds the exchange area");
+  static_assert(A_SE + 2 * N2 * T <= XD, "AV scratch exceeds the state area");
 };
 
 template <int N, int MODE>
@@ -205,16 +211,18 @@ __device__ __forceinline__ void face_flux(int info, int fam, int rfp, const Phys
 }
 
 template <int N, int MODE>
-__global__ __launch_bounds__(STAGE_THREADS) void stage_kernel(StageArgs a, Ops ops, Phys ph) {
+__global__ __launch_bounds__(STAGE_THREADS, 2) void stage_kernel(StageArgs a, Ops ops, Phys ph) {
   using G = Geo<N>;
   using LD = StageLds<N, MODE>;
   constexpr int T = G::CPW, N2 = G::N2, NF = N + 1, NT = 2 * N * T;
   constexpr bool AV = (MODE & M_AV) != 0, NORMS = (MODE & M_NORMS) != 0;
   constexpr bool COMPAT = (MODE & M_COMPAT) != 0, RK = (MODE & M_RK) != 0;
+  constexpr int NAUX = (MODE & M_AUX4) ? 4 : ((MODE & M_AUX1) ? 1 : 0);
+  constexpr bool RSTORE = (MODE & M_RSTORE) != 0;
   static_assert(NT <= STAGE_THREADS, "one thread per grid line");
   extern __shared__ double lds[];
   const int tid = threadIdx.x;
-  const int tile = tile_of_block(blockIdx.x, a.ntile);
+  const int tile = a.torder[tile_of_block(blockIdx.x, a.ntile)];
   const int ci = tid % T;
   const int fl = tid / T;              // 0..2N-1 for busy threads
   const int fam = fl >= N ? 1 : 0;     // 0: xi-line (row js), 1: eta-line (column is)
@@ -230,17 +238,32 @@ __global__ __launch_bounds__(STAGE_THREADS) void stage_kernel(StageArgs a, Ops o
   const int mx = fam ? G::S2X : G::S1X, my = fam ? G::S2Y : G::S1Y;
   const double *ql = lds + LD::Q0 + p0 * T + ci;   // line values: ql[(v*N2 + ps*s)*T]
 
+  // ---- the tile's state -> LDS in 16-byte loads (one contiguous block; all
+  //      loads are issued before the first LDS store) -------------------------
+  {
+    constexpr int NV2 = 9 * N2 * T / 2;                  // double2 per tile
+    constexpr int NST = (NV2 + STAGE_THREADS - 1) / STAGE_THREADS;
+    static_assert((G::QT * 8) % 16 == 0, "tiles start on 16-byte boundaries");
+    const double2 *src = reinterpret_cast<const double2 *>(a.qin + (size_t)tile * G::QT);
+    double2 *dst = reinterpret_cast<double2 *>(lds + LD::Q0);
+    // rounds k < NST-1 are full for every thread (tid < STAGE_THREADS), so
+    // only the last one is guarded and the copy stays one basic block
+    double2 st[NST];
+#pragma unroll
+    for (int k = 0; k < NST; ++k) {
+      const int i = tid + k * STAGE_THREADS;
+      st[k] = src[(k < NST - 1 || i < NV2) ? i : 0];
+    }
+#pragma unroll
+    for (int k = 0; k < NST - 1; ++k) dst[tid + k * STAGE_THREADS] = st[k];
+    if (tid + (NST - 1) * STAGE_THREADS < NV2) dst[tid + (NST - 1) * STAGE_THREADS] = st[NST - 1];
+  }
   // ---- gathers of the neighbour traces at both end faces --------------------
   double qn0[9], qnN[9];
   int info0 = 0, infoN = 0;
   if (valid) {
     info0 = face_gather<N>(a, ops, tile, ci, fam, line, 0, qn0);
     infoN = face_gather<N>(a, ops, tile, ci, fam, line, N, qnN);
-  }
-  // ---- the tile's state -> LDS (one contiguous block) -------------------------
-  {
-    const double *src = a.qin + (size_t)tile * G::QT;
-    for (int i = tid; i < 9 * N2 * T; i += STAGE_THREADS) lds[LD::Q0 + i] = src[i];
   }
   __syncthreads();
 
@@ -325,16 +348,57 @@ __global__ __launch_bounds__(STAGE_THREADS) void stage_kernel(StageArgs a, Ops o
       }
     }
     __builtin_amdgcn_sched_barrier(0);
-    if (fam == 1) {
+  }
+
+  // ---- RK register of the stage update at the xi-line's points, loaded now
+  //      so that its latency hides behind the exchange below (the output may
+  //      be the same array, so these loads cannot move past its stores) -------
+  //      (the first NPF points; each later point is loaded NPF points ahead
+  //      of its use, before the stores of the point in hand)
+  constexpr int NPF = (RK && NAUX == 1) ? ((AV || NORMS) ? 2 : 3) : 1;
+  double ax[NPF][9];
+  const double *aux0 = a.aux[0] + (size_t)tile * G::QT + ci;
+  if (RK && NAUX == 1 && valid && fam == 0) {
+#pragma unroll
+    for (int s = 0; s < NPF; ++s)
+#pragma unroll
+      for (int v = 0; v < 9; ++v) ax[s][v] = aux0[(v * N2 + N * line + s) * T];
+  }
+
+  // ---- the eta-line threads hand their derivatives to the xi-line threads,
+  //      XV variables per round; afterwards D, db, dj of a xi-line thread hold
+  //      the sum of both families at its points ---------------------------------
+  constexpr int XV = LD::XV;
+#pragma unroll
+  for (int r0 = 0; r0 < 9; r0 += XV) {
+    if (valid && fam == 1) {
 #pragma unroll
       for (int s = 0; s < N; ++s) {
         const int p = p0 + ps * s;
 #pragma unroll
-        for (int v = 0; v < 9; ++v) lds[LD::XD + (v * N2 + p) * T + ci] = D[s][v];
-        lds[LD::XB + p * T + ci] = db[s];
-        if (AV) lds[LD::XJ + p * T + ci] = dj[s];
+        for (int v = r0; v < r0 + XV && v < 9; ++v)
+          lds[LD::XD + ((v - r0) * N2 + p) * T + ci] = D[s][v];
+        if (r0 == 0) {
+          lds[LD::XB + p * T + ci] = db[s];
+          if (AV) lds[LD::XJ + p * T + ci] = dj[s];
+        }
       }
     }
+    __syncthreads();
+    if (valid && fam == 0) {
+#pragma unroll
+      for (int s = 0; s < N; ++s) {
+        const int p = N * line + s;
+#pragma unroll
+        for (int v = r0; v < r0 + XV && v < 9; ++v)
+          D[s][v] += lds[LD::XD + ((v - r0) * N2 + p) * T + ci];
+        if (r0 == 0) {
+          db[s] += lds[LD::XB + p * T + ci];
+          if (AV) dj[s] += lds[LD::XJ + p * T + ci];
+        }
+      }
+    }
+    if (r0 + XV < 9) __syncthreads();
   }
 
   // all-variable gradient nablaQs for the compresid output: the xi-line
@@ -368,9 +432,11 @@ __global__ __launch_bounds__(STAGE_THREADS) void stage_kernel(StageArgs a, Ops o
       }
     }
   };
-  if (COMPAT && valid && fam == 0) compat_gradient(false);
-  __syncthreads();
-  if (COMPAT && valid && fam == 1) compat_gradient(true);
+  if (COMPAT) {
+    if (valid && fam == 0) compat_gradient(false);
+    __syncthreads();
+    if (valid && fam == 1) compat_gradient(true);
+  }
 
   // ---- solution points of the xi-lines: point (is, js) = (s, line) ---------
   double rho_s[N], jz2_s[N];
@@ -384,9 +450,9 @@ __global__ __launch_bounds__(STAGE_THREADS) void stage_kernel(StageArgs a, Ops o
       for (int v = 0; v < 9; ++v) {
         q[v] = lds[LD::Q0 + (v * N2 + p) * T + ci];
         // resid = -(dF/dxi + dG/deta) / J   (compresid.f90:41,56)
-        R[v] = -(D[s][v] + lds[LD::XD + (v * N2 + p) * T + ci]) * ji;
+        R[v] = -D[s][v] * ji;
       }
-      const double divB = (db[s] + lds[LD::XB + p * T + ci]) * ji;
+      const double divB = db[s] * ji;
       const double ri = 1.0 / q[0];
       const double ux = q[1] * ri, uy = q[2] * ri, uz = q[3] * ri;
       // Powell source and GLM damping, compresid.f90:58-74
@@ -399,7 +465,7 @@ __global__ __launch_bounds__(STAGE_THREADS) void stage_kernel(StageArgs a, Ops o
       R[7] -= divB * uz;
       R[8] -= ph.alpha * q[8];
       if (AV) {
-        const double jz = (dj[s] + lds[LD::XJ + p * T + ci]) * ji;
+        const double jz = dj[s] * ji;
         rho_s[s] = q[0];
         jz2_s[s] = jz * jz;
       }
@@ -415,17 +481,37 @@ __global__ __launch_bounds__(STAGE_THREADS) void stage_kernel(StageArgs a, Ops o
       }
       if (RK) {
         const size_t off = (size_t)tile * G::QT + ci;
+        // all RK-register values of the point are read before its stores
+        // several registers (FiveStageRk's last stage): their weighted sum,
+        // formed before the point's stores
+        double xs[9];
+        if (NAUX > 1) {
+#pragma unroll
+          for (int v = 0; v < 9; ++v) xs[v] = 0.0;
+#pragma unroll
+          for (int i = 0; i < NAUX; ++i)
+#pragma unroll
+            for (int v = 0; v < 9; ++v)
+              xs[v] = fma(a.ca[i], a.aux[i][off + (size_t)(v * N2 + p) * T], xs[v]);
+        }
+        double xc[9];
+        if (NAUX == 1) {
+#pragma unroll
+          for (int v = 0; v < 9; ++v) xc[v] = ax[s % NPF][v];
+          if (s + NPF < N) {
+#pragma unroll
+            for (int v = 0; v < 9; ++v) ax[s % NPF][v] = aux0[(v * N2 + p + NPF) * T];
+          }
+        }
 #pragma unroll
         for (int v = 0; v < 9; ++v) {
           const size_t o = off + (size_t)(v * N2 + p) * T;
           double acc = a.cin * q[v];
-          if (a.naux > 0) acc = fma(a.ca[0], a.aux[0][o], acc);
-          if (a.naux > 1) acc = fma(a.ca[1], a.aux[1][o], acc);
-          if (a.naux > 2) acc = fma(a.ca[2], a.aux[2][o], acc);
-          if (a.naux > 3) acc = fma(a.ca[3], a.aux[3][o], acc);
+          if (NAUX == 1) acc = fma(a.ca[0], xc[v], acc);
+          if (NAUX > 1) acc += xs[v];
           acc = fma(a.cr, R[v], acc);
           a.qout[o] = acc;
-          if (a.rstore) a.rstore[o] = R[v];
+          if (RSTORE) a.rstore[o] = R[v];
         }
       }
     }
@@ -478,7 +564,7 @@ __global__ __launch_bounds__(STAGE_THREADS) void stage_kernel(StageArgs a, Ops o
     double *jz2 = lds + LD::A_JZ;            // [p][ci]
     double *lowb = lds + LD::A_LOW;          // [2][NL2][ci]
     double *seb = lds + LD::A_SE;            // [2][N2][ci]
-    // the derivative area is free: every xi-line thread has read its part
+    // the state area is free: every xi-line thread has read its points
     // before the reduction barrier above
     if (valid && fam == 0) {
 #pragma unroll

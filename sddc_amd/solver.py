@@ -18,7 +18,9 @@ from . import abi
 from . import mesh as M
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIBPATH = os.path.join(HERE, "lib", "libsd2d.so")
+# the library built by sddc_amd.build; SD2D_LIB names another build of the
+# same ABI (e.g. a variant under comparison)
+LIBPATH = os.environ.get("SD2D_LIB") or os.path.join(HERE, "lib", "libsd2d.so")
 _lib = None
 
 EXPORTS = ("sd2d_create", "sd2d_destroy", "sd2d_last_error", "sd2d_set_state",
