@@ -61,6 +61,8 @@ def main() -> int:
     ap.add_argument("--workload", default="c2", choices=["c1", "c2"])
     ap.add_argument("--cpu-steps", type=int, default=1,
                     help="steps of the CPU baseline sample (0 = skip)")
+    ap.add_argument("--no-mixed", action="store_true",
+                    help="skip the mixed-precision comparison")
     args = ap.parse_args()
     if args.gpus != 1:
         sys.stderr.write("bench.py measures one GPU only: this path runs the whole mesh on "
@@ -98,6 +100,8 @@ def main() -> int:
     balg = cases.alg_bytes_per_dof_stage(N, case.params.k_stage)
     achieved = ndof * balg / avg_s / 1e9
     d = g.diag()
+    nsteps_done = args.warmup + 2 * args.steps
+    q64 = g.get_state()[0] if not args.no_mixed else None
 
     res = {
         "metric": METRIC,
@@ -124,6 +128,44 @@ def main() -> int:
                  "eigvmax_g": d["eigvmax_g"]},
     }
     g.close()
+
+    if not args.no_mixed:
+        # the same workload with SD2D_MIXED: throughput, stage-kernel time and
+        # the relative L2 difference from the FP64 state after the same steps
+        import numpy as np
+        from sddc_amd import abi
+        gm = Sd2d(case, precision=abi.SD2D_MIXED)
+        gm.set_state(Q0, [0.0, 0.0, 0.0, 0.0])
+        gm.step(args.warmup, 10)
+        tm0 = time.perf_counter()
+        gm.step(args.steps, 10)
+        wall_m = time.perf_counter() - tm0
+        gm.set_profiling(True)
+        gm.step(args.steps, 10)
+        ms_m, n_m = gm.get_profile()
+        gm.set_profiling(False)
+        qm = gm.get_state()[0]
+        gm.close()
+        a = qm.reshape(9, -1, order="F"); b = q64.reshape(9, -1, order="F")
+        nb = np.sqrt((b * b).sum(axis=1))
+        den = nb.copy()
+        den[[3, 7, 8]] = nb.max()      # w, Bz, psi: against the largest variable
+        l2 = np.sqrt(((a - b) ** 2).sum(axis=1)) / den
+        avg_m = ms_m / 1e3 / max(n_m, 1)
+        balg_m = cases.alg_bytes_per_dof_stage(N, case.params.k_stage, metric_bytes=4)
+        vm = ndof * 5 * args.steps / wall_m
+        res["mixed"] = {
+            "value": vm, "unit": "DOF-stage/s", "ms_per_step": wall_m / args.steps * 1e3,
+            "speedup_vs_fp64": vm / value, "stage_kernel_ms": avg_m * 1e3,
+            "bytes_per_dof_stage": balg_m,
+            "roofline_frac": ndof * balg_m / avg_m / 1e9 / HBM_PEAK_GBS,
+            "l2_rel_vs_fp64": [float(x) for x in l2], "steps_compared": nsteps_done,
+            "l2_definition": "per variable ||q_mixed - q_fp64||_2 / ||q_fp64||_2; for w, Bz "
+                             "and psi the denominator is the largest variable's norm",
+            "note": "interpolation, fluxes, Rusanov and derivatives in FP32; state, "
+                    "sources, RK combination, norms and AV sensor in FP64"}
+        print(f"[bench] mixed: {vm / 1e9:.2f} G DOF-stage/s, x{vm / value:.2f}, "
+              f"max L2 rel {l2.max():.1e}", file=sys.stderr, flush=True)
 
     if args.cpu_steps > 0:
         from oracle.oracle import Oracle, threads

@@ -41,7 +41,11 @@ enum {
 /* Opaque per-GPU context. */
 typedef struct sd2d_ctx sd2d_ctx;
 
-/* Precision of the device path. */
+/* Precision of the device path.  SD2D_MIXED: interpolation, fluxes,
+ * Rusanov and derivatives in FP32 (operators and metrics rounded to FP32);
+ * the state, the Jacobian scaling, the sources, the RK combination, the
+ * norms and the AV sensor in FP64.  It applies to sd2d_step;
+ * sd2d_compresid always evaluates in FP64. */
 enum { SD2D_FP64 = 0, SD2D_MIXED = 1 };
 
 /*
@@ -174,6 +178,25 @@ int sd2d_get_diag(sd2d_ctx *ctx, sd2d_diag *diag);
  */
 int sd2d_get_av(sd2d_ctx *ctx, double *sm_ind, double *muav,
                 double *epsilon0);
+
+/*
+ * Restart files in the reference's format (WRITE_ALL_DATA_BINARY_MPI,
+ * write_data.f90:44-74; READ_ALL_DATA_BINARY, read_data_restart.f90:35-67):
+ * int32 iter, float64 ctime, then one record per cell: int32 global cell
+ * number, float64 Q(9,N,N) of that cell; native byte order.
+ *
+ * sd2d_write_restart writes the device state with the context's iter and
+ * ctime.  icg[ic] is the global number of local cell ic (1-based, as
+ * IC2ICG); NULL means 1..NCELL.
+ *
+ * sd2d_read_restart loads such a file: every record's cell goes to local
+ * cell ICG-1 (one rank holds the whole mesh, so NCELLGLOB == NCELL), iter
+ * and ctime are taken from the header, and the running maxima are reset to
+ * 0 as INIT_SETUP does after a restart (initsetup.f90:154-167).  A file of
+ * the wrong size or a cell number outside 1..NCELL is SD2D_EINVAL.
+ */
+int sd2d_write_restart(sd2d_ctx *ctx, const char *path, const int32_t *icg);
+int sd2d_read_restart(sd2d_ctx *ctx, const char *path);
 
 /*
  * Measurement aid, no counterpart in the reference.  While profiling is on,

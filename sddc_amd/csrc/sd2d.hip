@@ -19,6 +19,7 @@
 #include "../../include/sd2d.h"
 #include "sd2d_host.hpp"
 #include "sd2d_stage.hpp"
+#include "sd2d_stage_mx.hpp"
 #include "sd2d_rkseq.hpp"
 
 using namespace sd2d;
@@ -29,6 +30,8 @@ struct sd2d_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   Ops ops{};
+  OpsF opsf{};                                  // FP32 operators (SD2D_MIXED)
+  float *metf = nullptr;                        // FP32 metrics (SD2D_MIXED)
   Phys ph{};
   double *q[3] = {nullptr, nullptr, nullptr};  // RK state buffers
   int cur = 0;                                  // buffer holding the state
@@ -161,7 +164,19 @@ __global__ void k_norm_reduce(const double *__restrict__ nb, int ntile,
 // stage launch
 // ---------------------------------------------------------------------------
 template <int N, int MODE>
+static hipError_t launch_stage_mx_t(sd2d_ctx *c, const StageArgs &a) {
+  constexpr size_t lds = stage_mx_lds_bytes<N, MODE>();
+  static_assert(lds <= 65536, "stage kernel LDS above 64 KiB");
+  hipLaunchKernelGGL((stage_kernel_mx<N, MODE>), dim3(c->ntile), dim3(STAGE_THREADS), lds,
+                     c->stream, a, c->opsf, c->ph, c->metf);
+  return hipGetLastError();
+}
+
+template <int N, int MODE>
 static hipError_t launch_stage_t(sd2d_ctx *c, const StageArgs &a) {
+  if constexpr ((MODE & M_COMPAT) == 0) {
+    if (c->p.precision == SD2D_MIXED) return launch_stage_mx_t<N, MODE>(c, a);
+  }
   constexpr size_t lds = stage_lds_bytes<N, MODE>();
   static_assert(lds <= 65536, "stage kernel LDS above 64 KiB");
   static_assert(2 * N * Geo<N>::CPW <= STAGE_THREADS, "one thread per grid line");
@@ -270,8 +285,8 @@ int sd2d_create(const sd2d_mesh *m, const sd2d_params *p, sd2d_ctx **out) {
   if (p->vismode == 1 && (p->mu != 0.0 || p->eta0 != 0.0))
     return fail(nullptr, SD2D_EUNSUPPORTED,
                 "viscous/resistive fluxes (mu, eta0 != 0) are not implemented on the device");
-  if (p->precision != SD2D_FP64)
-    return fail(nullptr, SD2D_EUNSUPPORTED, "only SD2D_FP64 is implemented");
+  if (p->precision != SD2D_FP64 && p->precision != SD2D_MIXED)
+    return fail(nullptr, SD2D_EINVAL, "precision must be SD2D_FP64 or SD2D_MIXED");
   sd2d_ctx *c = new (std::nothrow) sd2d_ctx();
   if (!c) return fail(nullptr, SD2D_ENOMEM, "host allocation failed");
   HostTables tb;
@@ -284,6 +299,11 @@ int sd2d_create(const sd2d_mesh *m, const sd2d_params *p, sd2d_ctx **out) {
   c->N = N; c->ncell = nc; c->cpw = CPW; c->ntile = tb.ntile; c->p = *p;
   c->ctime = 0.0; c->iter = 0;
   c->ops = tb.ops;
+  for (int i = 0; i <= NMAX; ++i)
+    for (int j = 0; j < NMAX; ++j) {
+      c->opsf.L[i][j] = (float)tb.ops.L[i][j];
+      c->opsf.M[i][j] = (float)tb.ops.M[i][j];
+    }
   c->ph = tb.ph;
   const int nt = c->ntile;
   const std::vector<int> &ft = tb.ft;
@@ -306,6 +326,11 @@ int sd2d_create(const sd2d_mesh *m, const sd2d_params *p, sd2d_ctx **out) {
   if (p->k_stage == 5)
     for (int i = 0; i < 4; ++i) CK(hipMalloc(&c->fsr[i], qn * sizeof(double)));
   CK(hipMalloc(&c->met, met.size() * sizeof(double)));
+  std::vector<float> metf;
+  if (p->precision == SD2D_MIXED) {
+    metf.assign(met.begin(), met.end());
+    CK(hipMalloc(&c->metf, metf.size() * sizeof(float)));
+  }
   CK(hipMalloc(&c->ftab, ft.size() * sizeof(int)));
   CK(hipMalloc(&c->torder, tb.torder.size() * sizeof(int)));
   CK(hipMalloc(&c->avtab, av.size() * sizeof(double)));
@@ -316,6 +341,9 @@ int sd2d_create(const sd2d_mesh *m, const sd2d_params *p, sd2d_ctx **out) {
   CK(hipMalloc(&c->normout, 2 * sizeof(double)));
   CK(hipMalloc(&c->stagef, (size_t)nc * 9 * N2 * sizeof(double)));
   CK(hipMemcpyAsync(c->met, met.data(), met.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  if (c->metf)
+    CK(hipMemcpyAsync(c->metf, metf.data(), metf.size() * sizeof(float), hipMemcpyHostToDevice,
+                      c->stream));
   CK(hipMemcpyAsync(c->ftab, ft.data(), ft.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
   CK(hipMemcpyAsync(c->torder, tb.torder.data(), tb.torder.size() * sizeof(int),
                     hipMemcpyHostToDevice, c->stream));
@@ -336,7 +364,7 @@ void sd2d_destroy(sd2d_ctx *c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   for (double *b : c->q) if (b) hipFree(b);
   for (double *b : c->fsr) if (b) hipFree(b);
-  void *ptrs[] = {c->met, c->ftab, c->torder, c->avtab, c->gmax, c->slots, c->sm_ind, c->normbuf,
+  void *ptrs[] = {c->met, c->metf, c->ftab, c->torder, c->avtab, c->gmax, c->slots, c->sm_ind, c->normbuf,
                   c->normout, c->stagef, c->nablaf};
   for (void *b : ptrs) if (b) hipFree(b);
   for (hipEvent_t e : c->ev) hipEventDestroy(e);
@@ -499,6 +527,64 @@ int sd2d_step(sd2d_ctx *c, int32_t nsteps, int32_t norm_every) {
   HIPCHK(c, hipStreamSynchronize(c->stream));
   if (c->prof) return collect_profile(c);
   return SD2D_OK;
+}
+
+int sd2d_write_restart(sd2d_ctx *c, const char *path, const int32_t *icg) {
+  if (!c || !path) return fail(c, SD2D_EINVAL, "NULL argument");
+  const size_t rec = (size_t)9 * c->N * c->N;
+  std::vector<double> q(rec * c->ncell);
+  int r = sd2d_get_state(c, q.data(), nullptr);
+  if (r) return r;
+  FILE *fh = std::fopen(path, "wb");
+  if (!fh) return fail(c, SD2D_EINVAL, std::string("cannot open ") + path);
+  bool ok = std::fwrite(&c->iter, sizeof(int32_t), 1, fh) == 1 &&
+            std::fwrite(&c->ctime, sizeof(double), 1, fh) == 1;
+  for (int ic = 0; ok && ic < c->ncell; ++ic) {
+    const int32_t g = icg ? icg[ic] : ic + 1;
+    ok = std::fwrite(&g, sizeof(int32_t), 1, fh) == 1 &&
+         std::fwrite(q.data() + rec * ic, sizeof(double), rec, fh) == rec;
+  }
+  ok = (std::fclose(fh) == 0) && ok;
+  if (!ok) return fail(c, SD2D_EINVAL, std::string("write failed: ") + path);
+  return SD2D_OK;
+}
+
+int sd2d_read_restart(sd2d_ctx *c, const char *path) {
+  if (!c || !path) return fail(c, SD2D_EINVAL, "NULL argument");
+  const size_t rec = (size_t)9 * c->N * c->N;
+  FILE *fh = std::fopen(path, "rb");
+  if (!fh) return fail(c, SD2D_EINVAL, std::string("cannot open ") + path);
+  std::fseek(fh, 0, SEEK_END);
+  const long size = std::ftell(fh);
+  std::fseek(fh, 0, SEEK_SET);
+  const long want = (long)(sizeof(int32_t) + sizeof(double) +
+                           (sizeof(int32_t) + rec * sizeof(double)) * (size_t)c->ncell);
+  if (size != want) {
+    std::fclose(fh);
+    return fail(c, SD2D_EINVAL, "restart file size does not match NCELL and N");
+  }
+  int32_t it = 0;
+  double t = 0.0;
+  std::vector<double> q(rec * c->ncell, 0.0);
+  std::vector<char> seen(c->ncell, 0);
+  bool ok = std::fread(&it, sizeof(int32_t), 1, fh) == 1 && std::fread(&t, sizeof(double), 1, fh) == 1;
+  for (int i = 0; ok && i < c->ncell; ++i) {
+    int32_t g = 0;
+    ok = std::fread(&g, sizeof(int32_t), 1, fh) == 1;
+    if (!ok) break;
+    if (g < 1 || g > c->ncell || seen[g - 1]) {
+      std::fclose(fh);
+      return fail(c, SD2D_EINVAL, "restart record with a cell number outside 1..NCELL or repeated");
+    }
+    seen[g - 1] = 1;
+    ok = std::fread(q.data() + rec * (g - 1), sizeof(double), rec, fh) == rec;
+  }
+  std::fclose(fh);
+  if (!ok) return fail(c, SD2D_EINVAL, std::string("read failed: ") + path);
+  const sd2d_maxima zero{0.0, 0.0, 0.0, 0.0};
+  int r = sd2d_set_state(c, q.data(), &zero);
+  if (r) return r;
+  return sd2d_set_time(c, t, it);
 }
 
 int sd2d_set_profiling(sd2d_ctx *c, int32_t on) {

@@ -61,6 +61,11 @@ __host__ __device__ inline int finfo_pack(int kn, int left, int bc, int sneg) {
 
 constexpr double GAM = 5.0 / 3;   // setup2d.f90:67
 
+// 1/x: the correctly rounded quotient in FP64; in FP32 (the mixed-precision
+// path) the hardware reciprocal, accurate to 1 ulp.
+__device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
+__device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
 // ---------------------------------------------------------------------------
 // getIfluxvectors (getrusanov.f90:127-193) contracted with the metric
 // vector (sx, sy) of the flux point, as inviscid_flux_interior stores it
@@ -71,7 +76,7 @@ template <typename T>
 __device__ __forceinline__ void iflux_n(const T *q, T c_h, T sx, T sy, T *Fn, T &um) {
   const T rho = q[0], rhou = q[1], rhov = q[2], rhow = q[3], rhoE = q[4];
   const T Bx = q[5], By = q[6], Bz = q[7], psi = q[8];
-  const T ri = T(1) / rho;
+  const T ri = rcp(rho);
   const T u = rhou * ri, v = rhov * ri, w = rhow * ri;
   const T B2 = Bx * Bx + By * By + Bz * Bz;
   const T pr = T(GAM - 1.0) *
@@ -103,9 +108,9 @@ __device__ __forceinline__ void rusanov(const T *QL, const T *QR, T nx, T ny,
                                         int sign_l, T c_h, T *Fn, T &eig,
                                         T &um) {
   const T mag = sqrt(nx * nx + ny * ny);
-  const T im = T(sign_l) / mag;
+  const T im = T(sign_l) * rcp(mag);
   const T ux = nx * im, uy = ny * im;
-  const T ril = T(1) / QL[0], rir = T(1) / QR[0];
+  const T ril = rcp(QL[0]), rir = rcp(QR[0]);
   const T ul = ril * QL[1], vl = ril * QL[2], wl = ril * QL[3];
   const T ur = rir * QR[1], vr = rir * QR[2], wr = rir * QR[3];
   const T B2l = QL[5] * QL[5] + QL[6] * QL[6] + QL[7] * QL[7];
@@ -141,7 +146,7 @@ __device__ __forceinline__ void rusanov(const T *QL, const T *QR, T nx, T ny,
   // fast magneto-acoustic speed from averaged B and rho (:92-105)
   const T bx = T(0.5) * (QL[5] + QR[5]), by = T(0.5) * (QL[6] + QR[6]),
           bz = T(0.5) * (QL[7] + QR[7]);
-  const T irs = T(1) / (QL[0] + QR[0]);
+  const T irs = rcp(QL[0] + QR[0]);
   const T ca = T(GAM) * (pgl + pgr) * irs;
   const T cd = T(2) * (bx * bx + by * by + bz * bz) * irs;
   const T bn = bx * ux + by * uy;
@@ -168,9 +173,9 @@ template <typename T>
 __device__ __forceinline__ void bc_ghost(int bc, const T *Ql, T nx, T ny,
                                          const Phys &ph, T *Qr) {
   const T g1 = T(GAM - 1.0);
-  const T ri = T(1) / Ql[0];
+  const T ri = rcp(Ql[0]);
   const T uw = Ql[1] * ri, vw = Ql[2] * ri, ww = Ql[3] * ri;
-  const T im = T(1) / sqrt(nx * nx + ny * ny);
+  const T im = rcp(sqrt(nx * nx + ny * ny));
   const T vn = (uw * nx + vw * ny) * im;
   const T ur = uw - T(2) * vn * nx * im, vr = vw - T(2) * vn * ny * im;  // reflected
   const T rinf = T(ph.rinf), bxi = T(ph.bxinf), byi = T(ph.byinf);
@@ -186,7 +191,7 @@ __device__ __forceinline__ void bc_ghost(int bc, const T *Ql, T nx, T ny,
   // SYMP (bcflux.f90:272-299)
   const T rs = rinf * T(2) - Ql[0];
   const T ms1 = rs * ur, ms2 = rs * vr, ms3 = -rs * ww;
-  const T e_sy = p_e + T(0.5) / rs * (ms1 * ms1 + ms2 * ms2 + ms3 * ms3) +
+  const T e_sy = p_e + T(0.5) * rcp(rs) * (ms1 * ms1 + ms2 * ms2 + ms3 * ms3) +
                  T(0.5) * (bxi * bxi + byi * byi + Ql[7] * Ql[7] + Ql[8] * Ql[8]);
   // WALL (bcflux.f90:196-227): slip reflection for vismode 0, mirror else
   const bool slip = ph.vismode == 0;

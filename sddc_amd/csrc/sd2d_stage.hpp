@@ -102,9 +102,9 @@ constexpr size_t stage_lds_bytes() { return sizeof(double) * StageLds<N, MODE>::
 // acc = sum_s q[s]*Lr[s], summed in order s = 0..N-1 with FMA.  The own
 // trace and the neighbour trace at a face use this same formula so that
 // both cells form bitwise the same face state.
-template <int N>
-__device__ __forceinline__ double line_trace(const double *vals, const double *Lr) {
-  double acc = 0.0;
+template <int N, typename R>
+__device__ __forceinline__ R line_trace(const R *vals, const R *Lr) {
+  R acc = R(0);
 #pragma unroll
   for (int s = 0; s < N; ++s) acc = fma(vals[s], Lr[s], acc);
   return acc;
@@ -138,9 +138,9 @@ __device__ __forceinline__ void end_face(int fam, int line, int rfp, int &k, int
 // periodic face, the neighbour's trace Qn at the shared face point (its
 // line through that point, interpolated with the same arithmetic as the
 // own trace: interfaceflux_all.f90:59-76).
-template <int N>
-__device__ __forceinline__ int face_gather(const StageArgs &a, const Ops &ops, int tile, int ci,
-                                           int fam, int line, int rfp, double *Qn) {
+template <int N, typename R, typename OP>
+__device__ __forceinline__ int face_gather(const StageArgs &a, const OP &ops, int tile, int ci,
+                                           int fam, int line, int rfp, R *Qn) {
   constexpr int T = Geo<N>::CPW, N2 = Geo<N>::N2;
   int k, nfp;
   end_face<N>(fam, line, rfp, k, nfp);
@@ -156,15 +156,15 @@ __device__ __forceinline__ int face_gather(const StageArgs &a, const Ops &ops, i
     else if (kn == 2) { p0 = N - m; stride = N; }
     else { p0 = N * (N - m); stride = 1; }
     const bool endN = (kn == 1 || kn == 2);
-    double Lr[N];
+    R Lr[N];
 #pragma unroll
     for (int s = 0; s < N; ++s) Lr[s] = endN ? ops.L[N][s] : ops.L[0][s];
 #pragma unroll
     for (int v = 0; v < 9; ++v) {
-      double vals[N];
+      R vals[N];
 #pragma unroll
-      for (int s = 0; s < N; ++s) vals[s] = a.qin[nbase + (v * N2 + p0 + stride * s) * T];
-      Qn[v] = line_trace<N>(vals, Lr);
+      for (int s = 0; s < N; ++s) vals[s] = R(a.qin[nbase + (v * N2 + p0 + stride * s) * T]);
+      Qn[v] = line_trace<N, R>(vals, Lr);
     }
   }
   return info;
@@ -174,17 +174,17 @@ __device__ __forceinline__ int face_gather(const StageArgs &a, const Ops &ops, i
 // with the gathered neighbour trace (interfaceflux_all.f90:31-106,
 // cyclocflux_all.f90:226-301) or with the boundary ghost (bcflux.f90).
 // Returns the flux stored for this cell and the averaged state Qv.
-template <int N>
+template <int N, typename R>
 __device__ __forceinline__ void face_flux(int info, int fam, int rfp, const Phys &ph,
-                                          double c_h, const double *Qo, const double *Qn,
-                                          double sx, double sy, double *Fh, double *Qv,
-                                          double &eig, double &um) {
+                                          R c_h, const R *Qo, const R *Qn,
+                                          R sx, R sy, R *Fh, R *Qv,
+                                          R &eig, R &um) {
   const int k = fam == 0 ? (rfp == 0 ? 3 : 1) : (rfp == 0 ? 0 : 2);
   const int bc = (info >> 3) & 7;
   const int own_sl = (k == 0 || k == 3) ? -1 : 1;  // sign_l if this cell is left
   if (bc == FT_INTERIOR) {
     const int kn = info & 3, left = (info >> 2) & 1, sneg = (info >> 6) & 1;
-    double QL[9], QR[9];
+    R QL[9], QR[9];
 #pragma unroll
     for (int v = 0; v < 9; ++v) {
       QL[v] = left ? Qo[v] : Qn[v];
@@ -192,21 +192,21 @@ __device__ __forceinline__ void face_flux(int info, int fam, int rfp, const Phys
     }
     const int sl = left ? own_sl : ((kn == 0 || kn == 3) ? -1 : 1);
     const int sr = left ? ((kn == 1 || kn == 2) ? -1 : 1) : ((k == 1 || k == 2) ? -1 : 1);
-    const double nsg = (left || !sneg) ? 1.0 : -1.0;
-    double Fl[9];
-    rusanov<double>(QL, QR, nsg * sx, nsg * sy, sl, c_h, Fl, eig, um);
-    const double fac = left ? 1.0 : double(sl * sr);
+    const R nsg = (left || !sneg) ? R(1) : R(-1);
+    R Fl[9];
+    rusanov<R>(QL, QR, nsg * sx, nsg * sy, sl, c_h, Fl, eig, um);
+    const R fac = left ? R(1) : R(sl * sr);
 #pragma unroll
     for (int v = 0; v < 9; ++v) {
       Fh[v] = fac * Fl[v];
-      Qv[v] = 0.5 * QL[v] + 0.5 * QR[v];
+      Qv[v] = R(0.5) * QL[v] + R(0.5) * QR[v];
     }
   } else {
-    double QR[9];
-    bc_ghost<double>(bc, Qo, sx, sy, ph, QR);
-    rusanov<double>(Qo, QR, sx, sy, own_sl, c_h, Fh, eig, um);
+    R QR[9];
+    bc_ghost<R>(bc, Qo, sx, sy, ph, QR);
+    rusanov<R>(Qo, QR, sx, sy, own_sl, c_h, Fh, eig, um);
 #pragma unroll
-    for (int v = 0; v < 9; ++v) Qv[v] = 0.5 * Qo[v] + 0.5 * QR[v];
+    for (int v = 0; v < 9; ++v) Qv[v] = R(0.5) * Qo[v] + R(0.5) * QR[v];
   }
 }
 
@@ -262,8 +262,8 @@ __global__ __launch_bounds__(STAGE_THREADS, 2) void stage_kernel(StageArgs a, Op
   double qn0[9], qnN[9];
   int info0 = 0, infoN = 0;
   if (valid) {
-    info0 = face_gather<N>(a, ops, tile, ci, fam, line, 0, qn0);
-    infoN = face_gather<N>(a, ops, tile, ci, fam, line, N, qnN);
+    info0 = face_gather<N, double>(a, ops, tile, ci, fam, line, 0, qn0);
+    infoN = face_gather<N, double>(a, ops, tile, ci, fam, line, N, qnN);
   }
   __syncthreads();
 
@@ -280,11 +280,11 @@ __global__ __launch_bounds__(STAGE_THREADS, 2) void stage_kernel(StageArgs a, Op
         double vals[N];
 #pragma unroll
         for (int s = 0; s < N; ++s) vals[s] = ql[(v * N2 + ps * s) * T];
-        t0[v] = line_trace<N>(vals, ops.L[0]);
+        t0[v] = line_trace<N, double>(vals, ops.L[0]);
       }
       const double sx = mt[(mx + NF * line) * T], sy = mt[(my + NF * line) * T];
       double Qv[9];
-      face_flux<N>(info0, fam, 0, ph, c_h, t0, qn0, sx, sy, F0, Qv, eig, um);
+      face_flux<N, double>(info0, fam, 0, ph, c_h, t0, qn0, sx, sy, F0, Qv, eig, um);
       // gradient weights of the averaged field (nablaQsp, compflux.f90:107-157:
       // divB = dBx/dx + dBy/dy, Jz = dBy/dx - dBx/dy)
       wb0 = Qv[5] * sx + Qv[6] * sy;
@@ -302,11 +302,11 @@ __global__ __launch_bounds__(STAGE_THREADS, 2) void stage_kernel(StageArgs a, Op
         double vals[N];
 #pragma unroll
         for (int s = 0; s < N; ++s) vals[s] = ql[(v * N2 + ps * s) * T];
-        tN[v] = line_trace<N>(vals, ops.L[N]);
+        tN[v] = line_trace<N, double>(vals, ops.L[N]);
       }
       const double sx = mt[(mx + N + NF * line) * T], sy = mt[(my + N + NF * line) * T];
       double FN[9], Qv[9];
-      face_flux<N>(infoN, fam, N, ph, c_h, tN, qnN, sx, sy, FN, Qv, eig, um);
+      face_flux<N, double>(infoN, fam, N, ph, c_h, tN, qnN, sx, sy, FN, Qv, eig, um);
       const double wbN = Qv[5] * sx + Qv[6] * sy;
       const double wjN = Qv[6] * sx - Qv[5] * sy;
       if (COMPAT) {
@@ -332,7 +332,7 @@ __global__ __launch_bounds__(STAGE_THREADS, 2) void stage_kernel(StageArgs a, Op
         double vals[N];
 #pragma unroll
         for (int s = 0; s < N; ++s) vals[s] = ql[(v * N2 + ps * s) * T];
-        tr[v] = line_trace<N>(vals, ops.L[r]);
+        tr[v] = line_trace<N, double>(vals, ops.L[r]);
       }
       const double sx = mt[(mx + r + NF * line) * T], sy = mt[(my + r + NF * line) * T];
       double F[9];
@@ -413,7 +413,7 @@ __global__ __launch_bounds__(STAGE_THREADS, 2) void stage_kernel(StageArgs a, Op
       qv[0] = qe[0][v];
       qv[N] = qe[1][v];
 #pragma unroll
-      for (int r = 1; r < N; ++r) qv[r] = line_trace<N>(vals, ops.L[r]);
+      for (int r = 1; r < N; ++r) qv[r] = line_trace<N, double>(vals, ops.L[r]);
 #pragma unroll
       for (int s = 0; s < N; ++s) {
         double ax = 0.0, ay = 0.0;
@@ -489,10 +489,12 @@ __global__ __launch_bounds__(STAGE_THREADS, 2) void stage_kernel(StageArgs a, Op
 #pragma unroll
           for (int v = 0; v < 9; ++v) xs[v] = 0.0;
 #pragma unroll
-          for (int i = 0; i < NAUX; ++i)
+          for (int i = 0; i < NAUX; ++i) {
 #pragma unroll
             for (int v = 0; v < 9; ++v)
               xs[v] = fma(a.ca[i], a.aux[i][off + (size_t)(v * N2 + p) * T], xs[v]);
+            __builtin_amdgcn_sched_barrier(0);   // one register's 9 loads in flight at a time
+          }
         }
         double xc[9];
         if (NAUX == 1) {

@@ -1,0 +1,383 @@
+// sd2d_stage_mx.hpp - the mixed-precision form of the stage kernel
+// (precision SD2D_MIXED).
+//
+// Same one-thread-per-grid-line structure as stage_kernel, with the split
+// of precisions the mixed form is defined by:
+//   FP32   interpolation to the flux points, the interior fluxes, the
+//          Rusanov face fluxes and boundary ghosts, the line derivatives
+//          and the divB / Jz weights (operators and metrics in FP32);
+//   FP64   the state in HBM, the Jacobian scaling of the residual, the
+//          Powell and GLM sources, the RK stage combination, the norms and
+//          the AV sensor.
+// The tile's state is staged in LDS as FP32 for the traces; the points step
+// reads the FP64 state from memory (one point ahead of its use), and the
+// FP32 staging area is reused for the exchange of the eta-line derivatives.
+// About 1.1 KB of LDS per cell and roughly half the registers of the FP64
+// kernel, so more workgroups share a CU.
+#pragma once
+#include "sd2d_stage.hpp"
+
+namespace sd2d {
+
+// FP32 copies of the operators (rounded from the FP64 Lmat/Mmat).
+struct OpsF {
+  float L[NMAX + 1][NMAX];
+  float M[NMAX + 1][NMAX];
+};
+
+template <int N, int MODE> struct StageLdsMx {
+  static constexpr int T = Geo<N>::CPW, N2 = N * N, NL = N - 1;
+  static constexpr bool AV = (MODE & M_AV) != 0;
+  // in floats: the staged state [v][p][ci], reused for the eta-line flux
+  // derivatives; then the divB and Jz derivatives [p][ci]
+  static constexpr int QF = 0;
+  static constexpr int XB = 9 * N2 * T;
+  static constexpr int XJ = XB + N2 * T;
+  static constexpr int FLOATS = XJ + (AV ? N2 * T : 0);
+  // in doubles, after the float area: reduction slots, and the AV sensor
+  // scratch (rho, Jz^2, low-order projections, point errors)
+  static constexpr int D0 = (FLOATS + 1) / 2;
+  static constexpr int RED = D0;
+  static constexpr int A_RHO = RED + 4 * (STAGE_THREADS / 64);
+  static constexpr int A_JZ = A_RHO + N2 * T;
+  static constexpr int A_LOW = A_JZ + N2 * T;
+  static constexpr int A_SE = A_LOW + 2 * NL * NL * T;
+  static constexpr int TOTAL_D = AV ? A_SE + 2 * N2 * T : A_RHO;
+};
+
+template <int N, int MODE>
+constexpr size_t stage_mx_lds_bytes() { return sizeof(double) * StageLdsMx<N, MODE>::TOTAL_D; }
+
+// three waves per SIMD for the common stages; the rarer ones (AV sensor,
+// norms, FiveStageRk's residual store and four registers) carry more live
+// values and keep two
+template <int N, int MODE>
+__global__ __launch_bounds__(STAGE_THREADS,
+                             (MODE & (M_AV | M_NORMS | M_AUX4 | M_RSTORE)) ? 2 : 3)
+void stage_kernel_mx(StageArgs a, OpsF ops,
+                                                                       Phys ph,
+                                                                       const float *__restrict__ metf) {
+  using G = Geo<N>;
+  using LD = StageLdsMx<N, MODE>;
+  constexpr int T = G::CPW, N2 = G::N2, NF = N + 1, NT = 2 * N * T;
+  constexpr bool AV = (MODE & M_AV) != 0, NORMS = (MODE & M_NORMS) != 0;
+  constexpr bool RK = (MODE & M_RK) != 0;
+  constexpr int NAUX = (MODE & M_AUX4) ? 4 : ((MODE & M_AUX1) ? 1 : 0);
+  constexpr bool RSTORE = (MODE & M_RSTORE) != 0;
+  static_assert(RK && (MODE & M_COMPAT) == 0, "the mixed kernel is the stepping kernel");
+  static_assert(NT <= STAGE_THREADS, "one thread per grid line");
+  extern __shared__ double lds[];
+  float *ldf = reinterpret_cast<float *>(lds);
+  const int tid = threadIdx.x;
+  const int tile = a.torder[tile_of_block(blockIdx.x, a.ntile)];
+  const int ci = tid % T;
+  const int fl = tid / T;
+  const int fam = fl >= N ? 1 : 0;
+  const int line = fl - fam * N;
+  const int cell = tile * T + ci;
+  const bool valid = tid < NT && cell < a.ncell;
+  const double ge = a.gmax[0], gu = a.gmax[1];
+  const float c_h = float(sqrt(ge * (ge - gu)));   // compflux.f90:6
+  float eig = 0.f, um = 0.f;
+  double nres = 0.0, ndiv = 0.0;
+  const float *mtf = metf + (size_t)tile * G::NM * T + ci;
+  const double *mt = a.met + (size_t)tile * G::NM * T + ci;
+  const int p0 = fam ? line : N * line;
+  const int ps = fam ? N : 1;
+  const int mx = fam ? G::S2X : G::S1X, my = fam ? G::S2Y : G::S1Y;
+  const float *ql = ldf + LD::QF + p0 * T + ci;
+
+  // ---- the tile's state -> LDS as FP32 --------------------------------------
+  {
+    constexpr int NV2 = 9 * N2 * T / 2;
+    constexpr int NST = (NV2 + STAGE_THREADS - 1) / STAGE_THREADS;
+    const double2 *src = reinterpret_cast<const double2 *>(a.qin + (size_t)tile * G::QT);
+    float2 *dst = reinterpret_cast<float2 *>(ldf + LD::QF);
+    double2 st[NST];
+#pragma unroll
+    for (int k = 0; k < NST; ++k) {
+      const int i = tid + k * STAGE_THREADS;
+      st[k] = src[(k < NST - 1 || i < NV2) ? i : 0];
+    }
+#pragma unroll
+    for (int k = 0; k < NST; ++k) {
+      const int i = tid + k * STAGE_THREADS;
+      if (k < NST - 1 || i < NV2) dst[i] = make_float2(float(st[k].x), float(st[k].y));
+    }
+  }
+  // ---- gathers of the neighbour traces at both end faces --------------------
+  float qn0[9], qnN[9];
+  int info0 = 0, infoN = 0;
+  if (valid) {
+    info0 = face_gather<N, float>(a, ops, tile, ci, fam, line, 0, qn0);
+    infoN = face_gather<N, float>(a, ops, tile, ci, fam, line, N, qnN);
+  }
+  __syncthreads();
+
+  float D[N][9], db[N], dj[N];
+  if (valid) {
+    float F0[9], wb0, wj0;
+    {
+      float t0[9];
+#pragma unroll
+      for (int v = 0; v < 9; ++v) {
+        float vals[N];
+#pragma unroll
+        for (int s = 0; s < N; ++s) vals[s] = ql[(v * N2 + ps * s) * T];
+        t0[v] = line_trace<N, float>(vals, ops.L[0]);
+      }
+      const float sx = mtf[(mx + NF * line) * T], sy = mtf[(my + NF * line) * T];
+      float Qv[9];
+      face_flux<N, float>(info0, fam, 0, ph, c_h, t0, qn0, sx, sy, F0, Qv, eig, um);
+      wb0 = Qv[5] * sx + Qv[6] * sy;
+      wj0 = Qv[6] * sx - Qv[5] * sy;
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    {
+      float tN[9];
+#pragma unroll
+      for (int v = 0; v < 9; ++v) {
+        float vals[N];
+#pragma unroll
+        for (int s = 0; s < N; ++s) vals[s] = ql[(v * N2 + ps * s) * T];
+        tN[v] = line_trace<N, float>(vals, ops.L[N]);
+      }
+      const float sx = mtf[(mx + N + NF * line) * T], sy = mtf[(my + N + NF * line) * T];
+      float FN[9], Qv[9];
+      face_flux<N, float>(infoN, fam, N, ph, c_h, tN, qnN, sx, sy, FN, Qv, eig, um);
+      const float wbN = Qv[5] * sx + Qv[6] * sy;
+      const float wjN = Qv[6] * sx - Qv[5] * sy;
+#pragma unroll
+      for (int s = 0; s < N; ++s) {
+#pragma unroll
+        for (int v = 0; v < 9; ++v) D[s][v] = fmaf(FN[v], ops.M[N][s], F0[v] * ops.M[0][s]);
+        db[s] = fmaf(wbN, ops.M[N][s], wb0 * ops.M[0][s]);
+        dj[s] = fmaf(wjN, ops.M[N][s], wj0 * ops.M[0][s]);
+      }
+    }
+#pragma unroll
+    for (int r = 1; r < N; ++r) {
+      __builtin_amdgcn_sched_barrier(0);
+      float tr[9];
+#pragma unroll
+      for (int v = 0; v < 9; ++v) {
+        float vals[N];
+#pragma unroll
+        for (int s = 0; s < N; ++s) vals[s] = ql[(v * N2 + ps * s) * T];
+        tr[v] = line_trace<N, float>(vals, ops.L[r]);
+      }
+      const float sx = mtf[(mx + r + NF * line) * T], sy = mtf[(my + r + NF * line) * T];
+      float F[9];
+      iflux_n<float>(tr, c_h, sx, sy, F, um);
+      const float wb = tr[5] * sx + tr[6] * sy;
+      const float wj = tr[6] * sx - tr[5] * sy;
+#pragma unroll
+      for (int s = 0; s < N; ++s) {
+#pragma unroll
+        for (int v = 0; v < 9; ++v) D[s][v] = fmaf(F[v], ops.M[r][s], D[s][v]);
+        db[s] = fmaf(wb, ops.M[r][s], db[s]);
+        dj[s] = fmaf(wj, ops.M[r][s], dj[s]);
+      }
+    }
+    __builtin_amdgcn_sched_barrier(0);
+  }
+
+  // ---- FP64 state and RK register at the xi-line's first point, loaded now;
+  //      every later point one point ahead of its use, before the stores of
+  //      the point in hand ----------------------------------------------------------
+  const double *qg = a.qin + (size_t)tile * G::QT + ci + N * line * T;
+  const double *aux0 = (NAUX == 1 ? a.aux[0] : a.qin) + (size_t)tile * G::QT + ci + N * line * T;
+  double qn[9], xn[9];
+  if (valid && fam == 0) {
+#pragma unroll
+    for (int v = 0; v < 9; ++v) {
+      qn[v] = qg[v * N2 * T];
+      xn[v] = NAUX == 1 ? aux0[v * N2 * T] : 0.0;
+    }
+  }
+  __syncthreads();   // every thread is done with the staged state
+
+  // ---- the eta-line threads hand their derivatives to the xi-line threads ----
+  if (valid && fam == 1) {
+#pragma unroll
+    for (int s = 0; s < N; ++s) {
+      const int p = p0 + ps * s;
+#pragma unroll
+      for (int v = 0; v < 9; ++v) ldf[LD::QF + (v * N2 + p) * T + ci] = D[s][v];
+      ldf[LD::XB + p * T + ci] = db[s];
+      if (AV) ldf[LD::XJ + p * T + ci] = dj[s];
+    }
+  }
+  __syncthreads();
+
+  // ---- solution points of the xi-lines, in FP64 ---------------------------------
+  double rho_s[N], jz2_s[N];
+  if (valid && fam == 0) {
+#pragma unroll
+    for (int s = 0; s < N; ++s) {
+      const int p = N * line + s;
+      const double ji = mt[(G::JI + line + N * s) * T];
+      double R[9], q[9], xc[9];
+#pragma unroll
+      for (int v = 0; v < 9; ++v) {
+        q[v] = qn[v];
+        xc[v] = xn[v];
+        // resid = -(dF/dxi + dG/deta) / J   (compresid.f90:41,56)
+        R[v] = -double(D[s][v] + ldf[LD::QF + (v * N2 + p) * T + ci]) * ji;
+      }
+      const double divB = double(db[s] + ldf[LD::XB + p * T + ci]) * ji;
+      if (s + 1 < N) {
+#pragma unroll
+        for (int v = 0; v < 9; ++v) {
+          qn[v] = qg[(v * N2 + s + 1) * T];
+          if (NAUX == 1) xn[v] = aux0[(v * N2 + s + 1) * T];
+        }
+      }
+      const double ri = 1.0 / q[0];
+      const double ux = q[1] * ri, uy = q[2] * ri, uz = q[3] * ri;
+      // Powell source and GLM damping, compresid.f90:58-74
+      R[1] -= divB * q[5];
+      R[2] -= divB * q[6];
+      R[3] -= divB * q[7];
+      R[4] -= divB * (ux * q[5] + uy * q[6] + uz * q[7]);
+      R[5] -= divB * ux;
+      R[6] -= divB * uy;
+      R[7] -= divB * uz;
+      R[8] -= ph.alpha * q[8];
+      if (AV) {
+        const double jz = double(dj[s] + ldf[LD::XJ + p * T + ci]) * ji;
+        rho_s[s] = q[0];
+        jz2_s[s] = jz * jz;
+      }
+      if (NORMS) {
+#pragma unroll
+        for (int v = 0; v < 9; ++v) nres = fma(R[v], R[v], nres);
+        ndiv = fma(divB, divB, ndiv);
+      }
+      const size_t off = (size_t)tile * G::QT + ci;
+      double xs[9];
+      if (NAUX > 1) {
+#pragma unroll
+        for (int v = 0; v < 9; ++v) xs[v] = 0.0;
+#pragma unroll
+        for (int i = 0; i < NAUX; ++i) {
+#pragma unroll
+          for (int v = 0; v < 9; ++v)
+            xs[v] = fma(a.ca[i], a.aux[i][off + (size_t)(v * N2 + p) * T], xs[v]);
+          __builtin_amdgcn_sched_barrier(0);   // one register's 9 loads in flight at a time
+        }
+      }
+#pragma unroll
+      for (int v = 0; v < 9; ++v) {
+        const size_t o = off + (size_t)(v * N2 + p) * T;
+        double acc = a.cin * q[v];
+        if (NAUX == 1) acc = fma(a.ca[0], xc[v], acc);
+        if (NAUX > 1) acc += xs[v];
+        acc = fma(a.cr, R[v], acc);
+        a.qout[o] = acc;
+        if (RSTORE) a.rstore[o] = R[v];
+      }
+    }
+  }
+
+  // ---- block reductions -----------------------------------------------------------
+  {
+    constexpr int NW = STAGE_THREADS / 64;
+    const int w = tid >> 6, l = tid & 63;
+    double e = wave_max(double(eig)), u = wave_max(double(um));
+    if (NORMS) {
+      nres = wave_sum(nres);
+      ndiv = wave_sum(ndiv);
+    }
+    if (l == 0) {
+      lds[LD::RED + w] = e;
+      lds[LD::RED + NW + w] = u;
+      if (NORMS) {
+        lds[LD::RED + 2 * NW + w] = nres;
+        lds[LD::RED + 3 * NW + w] = ndiv;
+      }
+    }
+    __syncthreads();
+    if (tid == 0) {
+      double em = lds[LD::RED], ummax = lds[LD::RED + NW];
+      double s1 = NORMS ? lds[LD::RED + 2 * NW] : 0.0, s2 = NORMS ? lds[LD::RED + 3 * NW] : 0.0;
+#pragma unroll
+      for (int k = 1; k < NW; ++k) {
+        em = lds[LD::RED + k] > em ? lds[LD::RED + k] : em;
+        ummax = lds[LD::RED + NW + k] > ummax ? lds[LD::RED + NW + k] : ummax;
+        if (NORMS) {
+          s1 += lds[LD::RED + 2 * NW + k];
+          s2 += lds[LD::RED + 3 * NW + k];
+        }
+      }
+      unsigned long long *sl = a.slots + 2 * (blockIdx.x % NSLOT);
+      atomicMax(sl, (unsigned long long)__double_as_longlong(em));
+      atomicMax(sl + 1, (unsigned long long)__double_as_longlong(ummax));
+      if (NORMS) {
+        a.normbuf[2 * tile] = s1;
+        a.normbuf[2 * tile + 1] = s2;
+      }
+    }
+  }
+
+  // ---- AV smoothness sensor (AV.f90:14-83), in FP64 ---------------------------
+  if (AV) {
+    constexpr int NL = N - 1, NL2 = NL * NL;
+    const double *hlow = a.avtab;
+    const double *hhigh = a.avtab + NL * N;
+    double *rho = lds + LD::A_RHO, *jz2 = lds + LD::A_JZ;
+    double *lowb = lds + LD::A_LOW, *seb = lds + LD::A_SE;
+    if (valid && fam == 0) {
+#pragma unroll
+      for (int s = 0; s < N; ++s) {
+        rho[(N * line + s) * T + ci] = rho_s[s];
+        jz2[(N * line + s) * T + ci] = jz2_s[s];
+      }
+    }
+    __syncthreads();
+    const int nvc = a.ncell - tile * T;
+    for (int it = tid; it < NL2 * T; it += STAGE_THREADS) {
+      const int c = it % T, t = it / T;
+      const int il = t % NL, jl = t / NL;
+      if (c >= nvc) continue;
+      double s1 = 0.0, s2 = 0.0;
+      for (int js = 0; js < N; ++js)
+        for (int is = 0; is < N; ++is) {
+          const double h = hlow[il * N + is] * hlow[jl * N + js];
+          s1 = fma(rho[(is + N * js) * T + c], h, s1);
+          s2 = fma(jz2[(is + N * js) * T + c], h, s2);
+        }
+      lowb[(0 * NL2 + t) * T + c] = s1;
+      lowb[(1 * NL2 + t) * T + c] = s2;
+    }
+    __syncthreads();
+    for (int it = tid; it < N2 * T; it += STAGE_THREADS) {
+      const int c = it % T, p = it / T;
+      const int is = p % N, js = p / N;
+      if (c >= nvc) continue;
+      double h1 = 0.0, h2 = 0.0;
+      for (int jl = 0; jl < NL; ++jl)
+        for (int il = 0; il < NL; ++il) {
+          const double h = hhigh[is * NL + il] * hhigh[js * NL + jl];
+          h1 = fma(lowb[(0 * NL2 + il + NL * jl) * T + c], h, h1);
+          h2 = fma(lowb[(1 * NL2 + il + NL * jl) * T + c], h, h2);
+        }
+      const double r0 = rho[p * T + c], j0 = jz2[p * T + c];
+      const double d1 = h1 - r0, d2 = h2 - j0;
+      seb[(0 * N2 + p) * T + c] = (d1 * d1) / (r0 * r0);
+      seb[(1 * N2 + p) * T + c] = (d2 * d2) / (j0 * j0);
+    }
+    __syncthreads();
+    for (int c = tid; c < T && c < nvc; c += STAGE_THREADS) {
+      double s1 = 0.0, s2 = 0.0;
+      for (int p = 0; p < N2; ++p) {
+        s1 += seb[(0 * N2 + p) * T + c];
+        s2 += seb[(1 * N2 + p) * T + c];
+      }
+      a.sm_ind[2 * (size_t)(tile * T + c)] = log10(s1);
+      a.sm_ind[2 * (size_t)(tile * T + c) + 1] = log10(s2);
+    }
+  }
+}
+
+}  // namespace sd2d

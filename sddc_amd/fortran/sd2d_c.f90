@@ -56,8 +56,8 @@ module sd2d_c
 
   public :: sd2d_create, sd2d_destroy, sd2d_last_error, sd2d_set_state, &
        sd2d_get_state, sd2d_set_time, sd2d_compresid, sd2d_compresid_q, &
-       sd2d_step, sd2d_get_diag, sd2d_get_av, sd2d_set_profiling, &
-       sd2d_get_profile, sd2d_error_message
+       sd2d_step, sd2d_get_diag, sd2d_get_av, sd2d_write_restart, &
+       sd2d_read_restart, sd2d_set_profiling, sd2d_get_profile, sd2d_error_message
 
   interface
      integer(c_int) function sd2d_create(mesh, params, ctx) bind(C, name="sd2d_create")
@@ -133,6 +133,20 @@ module sd2d_c
        real(c_double), intent(out), optional :: muav(*)
        real(c_double), intent(out), optional :: epsilon0
      end function sd2d_get_av
+
+     ! path: a NUL-terminated string, e.g. trim(name)//c_null_char
+     integer(c_int) function sd2d_write_restart(ctx, path, icg) bind(C, name="sd2d_write_restart")
+       import :: c_int, c_ptr, c_char, c_int32_t
+       type(c_ptr), value :: ctx
+       character(kind=c_char), intent(in) :: path(*)
+       integer(c_int32_t), intent(in), optional :: icg(*)
+     end function sd2d_write_restart
+
+     integer(c_int) function sd2d_read_restart(ctx, path) bind(C, name="sd2d_read_restart")
+       import :: c_int, c_ptr, c_char
+       type(c_ptr), value :: ctx
+       character(kind=c_char), intent(in) :: path(*)
+     end function sd2d_read_restart
 
      integer(c_int) function sd2d_set_profiling(ctx, on) bind(C, name="sd2d_set_profiling")
        import :: c_int, c_ptr, c_int32_t

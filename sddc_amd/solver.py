@@ -26,6 +26,7 @@ _lib = None
 EXPORTS = ("sd2d_create", "sd2d_destroy", "sd2d_last_error", "sd2d_set_state",
            "sd2d_get_state", "sd2d_set_time", "sd2d_compresid",
            "sd2d_compresid_q", "sd2d_step", "sd2d_get_diag", "sd2d_get_av",
+           "sd2d_write_restart", "sd2d_read_restart",
            "sd2d_set_profiling", "sd2d_get_profile")
 
 
@@ -56,6 +57,8 @@ def load_library(path: str = LIBPATH):
     L.sd2d_step.argtypes = [vp, C.c_int32, C.c_int32]
     L.sd2d_get_diag.argtypes = [vp, C.POINTER(abi.Sd2dDiag)]
     L.sd2d_get_av.argtypes = [vp, vp, vp, C.POINTER(C.c_double)]
+    L.sd2d_write_restart.argtypes = [vp, C.c_char_p, vp]
+    L.sd2d_read_restart.argtypes = [vp, C.c_char_p]
     L.sd2d_set_profiling.argtypes = [vp, C.c_int32]
     L.sd2d_get_profile.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int32)]
     for f in EXPORTS:
@@ -131,6 +134,16 @@ class Sd2d:
         d = abi.Sd2dDiag()
         self._check(self.L.sd2d_get_diag(self.h, C.byref(d)), "sd2d_get_diag")
         return {k: getattr(d, k) for k, _ in abi.Sd2dDiag._fields_}
+
+    def write_restart(self, path, icg=None):
+        """Reference-format restart file of the device state."""
+        g = None if icg is None else np.ascontiguousarray(icg, dtype=np.int32)
+        self._check(self.L.sd2d_write_restart(self.h, str(path).encode(), _ptr(g)),
+                    "sd2d_write_restart")
+
+    def read_restart(self, path):
+        """Load a reference-format restart file (maxima reset to 0)."""
+        self._check(self.L.sd2d_read_restart(self.h, str(path).encode()), "sd2d_read_restart")
 
     def set_profiling(self, on: bool):
         self._check(self.L.sd2d_set_profiling(self.h, int(bool(on))), "sd2d_set_profiling")

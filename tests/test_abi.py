@@ -85,6 +85,8 @@ def test_null_arguments():
                     ("sd2d_step", (None, 1, 0)),
                     ("sd2d_get_diag", (None, None)),
                     ("sd2d_get_av", (None, None, None, None)),
+                    ("sd2d_write_restart", (None, b"x", None)),
+                    ("sd2d_read_restart", (None, b"x")),
                     ("sd2d_set_profiling", (None, 1)),
                     ("sd2d_get_profile", (None, None, None))]:
         assert getattr(L, f)(*args) == abi.SD2D_EINVAL, f
@@ -114,9 +116,9 @@ def test_viscous_rejected_on_device():
     assert rc == abi.SD2D_EUNSUPPORTED and "viscous" in msg
 
 
-def test_mixed_precision_not_yet():
-    rc, _, msg = _create(_small_case(), precision=abi.SD2D_MIXED)
-    assert rc == abi.SD2D_EUNSUPPORTED
+def test_unknown_precision():
+    rc, _, msg = _create(_small_case(), precision=7)
+    assert rc == abi.SD2D_EINVAL and "precision" in msg
 
 
 def test_incomplete_mesh():
