@@ -36,7 +36,47 @@ def build_workload(name: str):
         return cases.c2_cylinder(), "cylinder O-grid 512x1024 quads, r 0.5-30, p=4 (N=5), FP64, RK_S5P3"
     if name == "c1":
         return cases.c1_cylinder(), "cylinder O-grid 32x64 quads, r 0.5-15, p=3 (N=4), FP64, RK_S5P3"
+    if name == "c3":
+        return cases.orszag_tang(1024, N=4), ("periodic Orszag-Tang vortex 1024x1024 quads on "
+                                              "[0,2pi]^2, p=3 (N=4), RK_S5P3")
     raise SystemExit(f"unknown workload {name}")
+
+
+def mixed_to_time(case, Q0, t_end: float, desc: str) -> dict:
+    """FP64 and mixed runs of the same workload from t=0 to t_end: the time
+    of each and the per-variable relative L2 difference at t_end (w, Bz and
+    psi, which start at zero, against the largest variable's norm)."""
+    import numpy as np
+    from sddc_amd import abi, cases
+    from sddc_amd.solver import Sd2d
+    nsteps = int(np.ceil(t_end / case.params.dt - 1e-9))
+    ndof = cases.dof(case)
+    out, times = {}, {}
+    for prec in (abi.SD2D_FP64, abi.SD2D_MIXED):
+        g = Sd2d(case, precision=prec)
+        g.set_state(Q0, [0.0, 0.0, 0.0, 0.0])
+        g.step(1, 0)
+        t0 = time.perf_counter()
+        g.step(nsteps - 1, 0)
+        times[prec] = time.perf_counter() - t0
+        out[prec] = g.get_state()[0]
+        ctime = g.diag()["ctime"]
+        g.close()
+        print(f"[bench] precision {prec}: {nsteps} steps to t={ctime:.4f} in "
+              f"{times[prec]:.1f}s", file=sys.stderr, flush=True)
+    a = out[abi.SD2D_MIXED].reshape(9, -1, order="F")
+    b = out[abi.SD2D_FP64].reshape(9, -1, order="F")
+    nb = np.sqrt((b * b).sum(axis=1))
+    den = nb.copy()
+    den[[3, 7, 8]] = nb.max()
+    l2 = np.sqrt(((a - b) ** 2).sum(axis=1)) / den
+    v64 = ndof * 5 * (nsteps - 1) / times[abi.SD2D_FP64]
+    vmx = ndof * 5 * (nsteps - 1) / times[abi.SD2D_MIXED]
+    return {"metric": METRIC, "workload": desc, "t_end": t_end, "steps": nsteps,
+            "dt": case.params.dt, "dof": ndof,
+            "fp64_value": v64, "mixed_value": vmx, "unit": "DOF-stage/s",
+            "speedup": vmx / v64, "l2_rel_mixed_vs_fp64": [float(x) for x in l2],
+            "all_finite": bool(np.all(np.isfinite(a)))}
 
 
 def traffic_from_profiles(kernel_tag: str):
@@ -58,7 +98,10 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2"])
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3"])
+    ap.add_argument("--t-end", type=float, default=None,
+                    help="instead of the throughput run: FP64 and mixed runs to this "
+                         "time, reporting the mixed form's L2 difference")
     ap.add_argument("--cpu-steps", type=int, default=1,
                     help="steps of the CPU baseline sample (0 = skip)")
     ap.add_argument("--no-mixed", action="store_true",
@@ -77,6 +120,9 @@ def main() -> int:
     (case, Q0), desc = build_workload(args.workload)
     print(f"[bench] workload {args.workload}: {case.ncell} cells, N={case.N}, "
           f"built in {time.perf_counter() - t:.1f}s", file=sys.stderr, flush=True)
+    if args.t_end is not None:
+        print(json.dumps(mixed_to_time(case, Q0, args.t_end, desc)), flush=True)
+        return 0
     N = case.N
     ndof = cases.dof(case)
     g = Sd2d(case)

@@ -134,40 +134,80 @@ __device__ __forceinline__ void end_face(int fam, int line, int rfp, int &k, int
   nfp = fam == 0 ? (rfp == 0 ? N - line : line + 1) : (rfp == 0 ? line + 1 : N - line);
 }
 
+// The neighbour's line through the shared point of an end face: its first
+// point p0 and stride, and whether the point is the line's end N (otherwise
+// its end 0), from the neighbour's local face kn and the own point index
+// nfp (the neighbour walks the face the other way: interfaceflux_all.f90:59-60).
+template <int N>
+__device__ __forceinline__ void neighbour_line(int kn, int nfp, int &p0, int &stride, bool &endN) {
+  const int m = N - nfp + 1;  // neighbour point index on its face (1-based)
+  if (kn == 0) { p0 = m - 1; stride = N; }
+  else if (kn == 1) { p0 = N * (m - 1); stride = 1; }
+  else if (kn == 2) { p0 = N - m; stride = N; }
+  else { p0 = N * (N - m); stride = 1; }
+  endN = (kn == 1 || kn == 2);
+}
+
 // Gather for an end face: the face-table info word and, for an interior or
 // periodic face, the neighbour's trace Qn at the shared face point (its
 // line through that point, interpolated with the same arithmetic as the
-// own trace: interfaceflux_all.f90:59-76).
+// own trace: interfaceflux_all.f90:59-76).  When the neighbour cell is in
+// the same tile, nothing is read here: lci returns its index in the tile
+// and the trace is formed from the staged tile by face_trace_lds.
 template <int N, typename R, typename OP>
 __device__ __forceinline__ int face_gather(const StageArgs &a, const OP &ops, int tile, int ci,
-                                           int fam, int line, int rfp, R *Qn) {
-  constexpr int T = Geo<N>::CPW, N2 = Geo<N>::N2;
+                                           int fam, int line, int rfp, R *Qn, int &lci) {
+  constexpr int T = Geo<N>::CPW, N2 = Geo<N>::N2, QT = Geo<N>::QT;
   int k, nfp;
   end_face<N>(fam, line, rfp, k, nfp);
   const int *ft = a.ftab + ((size_t)tile * 8 + k * 2) * T + ci;
   const int nbase = ft[0];
   const int info = ft[T];
+  lci = -1;
   if (((info >> 3) & 7) == FT_INTERIOR) {
-    const int kn = info & 3;
-    const int m = N - nfp + 1;  // neighbour point index on its face (1-based)
-    int p0, stride;
-    if (kn == 0) { p0 = m - 1; stride = N; }
-    else if (kn == 1) { p0 = N * (m - 1); stride = 1; }
-    else if (kn == 2) { p0 = N - m; stride = N; }
-    else { p0 = N * (N - m); stride = 1; }
-    const bool endN = (kn == 1 || kn == 2);
-    R Lr[N];
+    const int ntile = nbase / QT;
+    if (ntile == tile) {
+      lci = nbase - tile * QT;
+    } else {
+      int p0, stride;
+      bool endN;
+      neighbour_line<N>(info & 3, nfp, p0, stride, endN);
+      R Lr[N];
 #pragma unroll
-    for (int s = 0; s < N; ++s) Lr[s] = endN ? ops.L[N][s] : ops.L[0][s];
+      for (int s = 0; s < N; ++s) Lr[s] = endN ? ops.L[N][s] : ops.L[0][s];
 #pragma unroll
-    for (int v = 0; v < 9; ++v) {
-      R vals[N];
+      for (int v = 0; v < 9; ++v) {
+        R vals[N];
 #pragma unroll
-      for (int s = 0; s < N; ++s) vals[s] = R(a.qin[nbase + (v * N2 + p0 + stride * s) * T]);
-      Qn[v] = line_trace<N, R>(vals, Lr);
+        for (int s = 0; s < N; ++s) vals[s] = R(a.qin[nbase + (v * N2 + p0 + stride * s) * T]);
+        Qn[v] = line_trace<N, R>(vals, Lr);
+      }
     }
   }
   return info;
+}
+
+// The neighbour's trace for a neighbour in the same tile, from the staged
+// tile (the same values and arithmetic as face_gather).
+template <int N, typename R, typename OP>
+__device__ __forceinline__ void face_trace_lds(const R *qs, const OP &ops, int info, int fam,
+                                               int line, int rfp, int lci, R *Qn) {
+  constexpr int T = Geo<N>::CPW, N2 = Geo<N>::N2;
+  int k, nfp;
+  end_face<N>(fam, line, rfp, k, nfp);
+  int p0, stride;
+  bool endN;
+  neighbour_line<N>(info & 3, nfp, p0, stride, endN);
+  R Lr[N];
+#pragma unroll
+  for (int s = 0; s < N; ++s) Lr[s] = endN ? ops.L[N][s] : ops.L[0][s];
+#pragma unroll
+  for (int v = 0; v < 9; ++v) {
+    R vals[N];
+#pragma unroll
+    for (int s = 0; s < N; ++s) vals[s] = qs[(v * N2 + p0 + stride * s) * T + lci];
+    Qn[v] = line_trace<N, R>(vals, Lr);
+  }
 }
 
 // Flux at an end face point: Rusanov in the reference's left/right roles
@@ -258,14 +298,18 @@ __global__ __launch_bounds__(STAGE_THREADS, 2) void stage_kernel(StageArgs a, Op
     for (int k = 0; k < NST - 1; ++k) dst[tid + k * STAGE_THREADS] = st[k];
     if (tid + (NST - 1) * STAGE_THREADS < NV2) dst[tid + (NST - 1) * STAGE_THREADS] = st[NST - 1];
   }
-  // ---- gathers of the neighbour traces at both end faces --------------------
+  // ---- neighbour traces at both end faces: gathered from memory for a
+  //      neighbour in another tile, formed from the staged tile after the
+  //      barrier for a neighbour in this one --------------------------------------
   double qn0[9], qnN[9];
-  int info0 = 0, infoN = 0;
+  int info0 = 0, infoN = 0, lci0 = -1, lciN = -1;
   if (valid) {
-    info0 = face_gather<N, double>(a, ops, tile, ci, fam, line, 0, qn0);
-    infoN = face_gather<N, double>(a, ops, tile, ci, fam, line, N, qnN);
+    info0 = face_gather<N, double>(a, ops, tile, ci, fam, line, 0, qn0, lci0);
+    infoN = face_gather<N, double>(a, ops, tile, ci, fam, line, N, qnN, lciN);
   }
   __syncthreads();
+  if (valid && lci0 >= 0) face_trace_lds<N, double>(lds + LD::Q0, ops, info0, fam, line, 0, lci0, qn0);
+  if (valid && lciN >= 0) face_trace_lds<N, double>(lds + LD::Q0, ops, infoN, fam, line, N, lciN, qnN);
 
   double D[N][9];            // d(flux)/d(line coordinate) at the line's points
   double db[N], dj[N];       // same for the divB and Jz weights

@@ -105,14 +105,16 @@ void stage_kernel_mx(StageArgs a, OpsF ops,
       if (k < NST - 1 || i < NV2) dst[i] = make_float2(float(st[k].x), float(st[k].y));
     }
   }
-  // ---- gathers of the neighbour traces at both end faces --------------------
+  // ---- neighbour traces at both end faces (memory or the staged tile) -------
   float qn0[9], qnN[9];
-  int info0 = 0, infoN = 0;
+  int info0 = 0, infoN = 0, lci0 = -1, lciN = -1;
   if (valid) {
-    info0 = face_gather<N, float>(a, ops, tile, ci, fam, line, 0, qn0);
-    infoN = face_gather<N, float>(a, ops, tile, ci, fam, line, N, qnN);
+    info0 = face_gather<N, float>(a, ops, tile, ci, fam, line, 0, qn0, lci0);
+    infoN = face_gather<N, float>(a, ops, tile, ci, fam, line, N, qnN, lciN);
   }
   __syncthreads();
+  if (valid && lci0 >= 0) face_trace_lds<N, float>(ldf + LD::QF, ops, info0, fam, line, 0, lci0, qn0);
+  if (valid && lciN >= 0) face_trace_lds<N, float>(ldf + LD::QF, ops, infoN, fam, line, N, lciN, qnN);
 
   float D[N][9], db[N], dj[N];
   if (valid) {
