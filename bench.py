@@ -209,7 +209,7 @@ def main() -> int:
             "l2_definition": "per variable ||q_mixed - q_fp64||_2 / ||q_fp64||_2; for w, Bz "
                              "and psi the denominator is the largest variable's norm",
             "note": "interpolation, fluxes, Rusanov and derivatives in FP32; state, "
-                    "sources, RK combination, norms and AV sensor in FP64"}
+                    "pressure recovery, sources, RK combination, norms and AV sensor in FP64"}
         print(f"[bench] mixed: {vm / 1e9:.2f} G DOF-stage/s, x{vm / value:.2f}, "
               f"max L2 rel {l2.max():.1e}", file=sys.stderr, flush=True)
 

@@ -20,6 +20,7 @@
 #include "sd2d_host.hpp"
 #include "sd2d_stage.hpp"
 #include "sd2d_stage_mx.hpp"
+#include "sd2d_visc.hpp"
 #include "sd2d_rkseq.hpp"
 
 using namespace sd2d;
@@ -46,6 +47,9 @@ struct sd2d_ctx {
   double *normbuf = nullptr, *normout = nullptr;
   double *stagef = nullptr;                     // Fortran-layout staging (Q or resid)
   double *nablaf = nullptr;                     // Fortran-layout nablaQs
+  // viscous path (vismode 1, mu or eta0 non-zero)
+  bool vis = false;
+  double *gq = nullptr, *dei = nullptr, *visres = nullptr, *metw = nullptr;
   double ctime = 0.0;
   int iter = 0;
   double last_c_h = 0.0;
@@ -174,7 +178,7 @@ static hipError_t launch_stage_mx_t(sd2d_ctx *c, const StageArgs &a) {
 
 template <int N, int MODE>
 static hipError_t launch_stage_t(sd2d_ctx *c, const StageArgs &a) {
-  if constexpr ((MODE & M_COMPAT) == 0) {
+  if constexpr ((MODE & (M_COMPAT | M_VIS)) == 0) {
     if (c->p.precision == SD2D_MIXED) return launch_stage_mx_t<N, MODE>(c, a);
   }
   constexpr size_t lds = stage_lds_bytes<N, MODE>();
@@ -203,6 +207,19 @@ static hipError_t launch_stage_n(sd2d_ctx *c, const StageArgs &a, int mode) {
     SD2D_MODE(R | A4 | V | NR)
     SD2D_MODE(M_COMPAT)
     SD2D_MODE(M_COMPAT | V)
+    // the same with the viscous divergence (FP64 only)
+    SD2D_MODE(M_VIS | R)
+    SD2D_MODE(M_VIS | R | A1)
+    SD2D_MODE(M_VIS | R | A1 | V)
+    SD2D_MODE(M_VIS | R | A1 | NR)
+    SD2D_MODE(M_VIS | R | A1 | V | NR)
+    SD2D_MODE(M_VIS | R | A1 | M_RSTORE)
+    SD2D_MODE(M_VIS | R | A4)
+    SD2D_MODE(M_VIS | R | A4 | V)
+    SD2D_MODE(M_VIS | R | A4 | NR)
+    SD2D_MODE(M_VIS | R | A4 | V | NR)
+    SD2D_MODE(M_VIS | M_COMPAT)
+    SD2D_MODE(M_VIS | M_COMPAT | V)
 #undef SD2D_MODE
     default: return hipErrorInvalidValue;
   }
@@ -213,6 +230,33 @@ static hipError_t launch_stage_raw(sd2d_ctx *c, const StageArgs &a, int mode) {
     case 3: return launch_stage_n<3>(c, a, mode);
     case 4: return launch_stage_n<4>(c, a, mode);
     default: return launch_stage_n<5>(c, a, mode);
+  }
+}
+
+// the viscous divergence of the residual of state qin -> c->visres
+template <int N>
+static hipError_t launch_vis_n(sd2d_ctx *c, const double *qin) {
+  VisArgs va{};
+  va.qin = qin; va.ftab = c->ftab; va.met = c->met; va.metw = c->metw;
+  va.gq = c->gq; va.dei = c->dei; va.visres = c->visres;
+  va.mu = c->p.mu; va.eta0 = c->p.eta0;
+  va.ncell = c->ncell; va.ntile = c->ntile; va.walls = c->dei != nullptr;
+  constexpr size_t lds = vis_lds_bytes<N>();
+  static_assert(lds <= 65536, "viscous kernel LDS above 64 KiB");
+  hipLaunchKernelGGL(vis_grad_kernel<N>, dim3(c->ntile), dim3(STAGE_THREADS), lds, c->stream,
+                     va, c->ops, c->ph);
+  hipError_t r = hipGetLastError();
+  if (r != hipSuccess) return r;
+  hipLaunchKernelGGL(vis_flux_kernel<N>, dim3(c->ntile), dim3(STAGE_THREADS), lds, c->stream,
+                     va, c->ops, c->ph);
+  return hipGetLastError();
+}
+
+static hipError_t launch_vis(sd2d_ctx *c, const double *qin) {
+  switch (c->N) {
+    case 3: return launch_vis_n<3>(c, qin);
+    case 4: return launch_vis_n<4>(c, qin);
+    default: return launch_vis_n<5>(c, qin);
   }
 }
 
@@ -282,11 +326,12 @@ int sd2d_create(const sd2d_mesh *m, const sd2d_params *p, sd2d_ctx **out) {
     return fail(nullptr, SD2D_EINVAL, "incomplete mesh");
   if (p->k_stage != 3 && p->k_stage != 5)
     return fail(nullptr, SD2D_EINVAL, "k_stage can be only 3 or 5");
-  if (p->vismode == 1 && (p->mu != 0.0 || p->eta0 != 0.0))
-    return fail(nullptr, SD2D_EUNSUPPORTED,
-                "viscous/resistive fluxes (mu, eta0 != 0) are not implemented on the device");
   if (p->precision != SD2D_FP64 && p->precision != SD2D_MIXED)
     return fail(nullptr, SD2D_EINVAL, "precision must be SD2D_FP64 or SD2D_MIXED");
+  const bool vis = p->vismode == 1 && (p->mu != 0.0 || p->eta0 != 0.0);
+  if (vis && p->precision == SD2D_MIXED)
+    return fail(nullptr, SD2D_EUNSUPPORTED,
+                "the viscous/resistive path (mu, eta0 != 0) is FP64 only");
   sd2d_ctx *c = new (std::nothrow) sd2d_ctx();
   if (!c) return fail(nullptr, SD2D_ENOMEM, "host allocation failed");
   HostTables tb;
@@ -305,6 +350,7 @@ int sd2d_create(const sd2d_mesh *m, const sd2d_params *p, sd2d_ctx **out) {
       c->opsf.M[i][j] = (float)tb.ops.M[i][j];
     }
   c->ph = tb.ph;
+  c->vis = vis;
   const int nt = c->ntile;
   const std::vector<int> &ft = tb.ft;
   const std::vector<double> &met = tb.met;
@@ -340,6 +386,15 @@ int sd2d_create(const sd2d_mesh *m, const sd2d_params *p, sd2d_ctx **out) {
   CK(hipMalloc(&c->normbuf, 2 * (size_t)nt * sizeof(double)));
   CK(hipMalloc(&c->normout, 2 * sizeof(double)));
   CK(hipMalloc(&c->stagef, (size_t)nc * 9 * N2 * sizeof(double)));
+  if (vis) {
+    CK(hipMalloc(&c->gq, 2 * qn * sizeof(double)));
+    CK(hipMalloc(&c->visres, qn * sizeof(double)));
+    CK(hipMalloc(&c->metw, tb.metw.size() * sizeof(double)));
+    CK(hipMemcpyAsync(c->metw, tb.metw.data(), tb.metw.size() * sizeof(double),
+                      hipMemcpyHostToDevice, c->stream));
+    if (m->nwall > 0)   // comp_grad_internal_energy runs only with walls (compflux.f90:16)
+      CK(hipMalloc(&c->dei, (size_t)nt * 2 * N2 * CPW * sizeof(double)));
+  }
   CK(hipMemcpyAsync(c->met, met.data(), met.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
   if (c->metf)
     CK(hipMemcpyAsync(c->metf, metf.data(), metf.size() * sizeof(float), hipMemcpyHostToDevice,
@@ -361,14 +416,14 @@ int sd2d_create(const sd2d_mesh *m, const sd2d_params *p, sd2d_ctx **out) {
 
 void sd2d_destroy(sd2d_ctx *c) {
   if (!c) return;
-  if (c->stream) hipStreamSynchronize(c->stream);
-  for (double *b : c->q) if (b) hipFree(b);
-  for (double *b : c->fsr) if (b) hipFree(b);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (double *b : c->q) if (b) (void)hipFree(b);
+  for (double *b : c->fsr) if (b) (void)hipFree(b);
   void *ptrs[] = {c->met, c->metf, c->ftab, c->torder, c->avtab, c->gmax, c->slots, c->sm_ind, c->normbuf,
-                  c->normout, c->stagef, c->nablaf};
-  for (void *b : ptrs) if (b) hipFree(b);
-  for (hipEvent_t e : c->ev) hipEventDestroy(e);
-  if (c->stream) hipStreamDestroy(c->stream);
+                  c->normout, c->stagef, c->nablaf, c->gq, c->dei, c->visres, c->metw};
+  for (void *b : ptrs) if (b) (void)hipFree(b);
+  for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
 
@@ -460,7 +515,12 @@ int sd2d_compresid(sd2d_ctx *c, double *resid_out, double *nablaqs_out) {
   a.qin = c->q[c->cur];
   a.resid_f = c->stagef;
   a.nabla_f = c->nablaf;
-  const int mode = M_COMPAT | (c->p.artiv == 1 && c->p.vismode == 1 ? M_AV : 0);
+  int mode = M_COMPAT | (c->p.artiv == 1 && c->p.vismode == 1 ? M_AV : 0);
+  if (c->vis) {
+    HIPCHK(c, launch_vis(c, a.qin));
+    a.visres = c->visres;
+    mode |= M_VIS;
+  }
   HIPCHK(c, launch_stage(c, a, mode));
   c->have_av = (mode & M_AV) != 0;
   r = finalize_stage(c);
@@ -484,6 +544,11 @@ static int step_rk(sd2d_ctx *c, bool norms) {
   const int av = (c->p.artiv == 1 && c->p.vismode == 1) ? M_AV : 0;
   auto launch = [&](StageArgs &a, int mode) -> int {
     if (mode & M_NORMS) a.normbuf = c->normbuf;
+    if (c->vis) {
+      HIPCHK(c, launch_vis(c, a.qin));
+      a.visres = c->visres;
+      mode |= M_VIS;
+    }
     HIPCHK(c, launch_stage(c, a, mode));
     return finalize_stage(c);
   };

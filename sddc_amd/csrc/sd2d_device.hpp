@@ -66,6 +66,19 @@ constexpr double GAM = 5.0 / 3;   // setup2d.f90:67
 __device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
 __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
+// Gas pressure (gamma-1)*(E - |m|^2/(2 rho) - |B|^2/2 - psi^2/2), the
+// recovery that cancels at low plasma beta.  The FP64 path keeps the
+// operation order of each caller (iflux_n, rusanov); the mixed path
+// (T = float) forms it in FP64 from the FP32 traces and rounds once, after
+// the cancellation.
+__device__ __forceinline__ float pressure_mx(float rho, float m1, float m2, float m3, float E,
+                                             float b1, float b2, float b3, float psi) {
+  const double r = rho, x = m1, y = m2, z = m3, bx = b1, by = b2, bz = b3, ps = psi;
+  const double ke = 0.5 * (x * x + y * y + z * z) / r;
+  const double me = 0.5 * (bx * bx + by * by + bz * bz);
+  return float((GAM - 1.0) * (((double(E) - ke) - me) - 0.5 * ps * ps));
+}
+
 // ---------------------------------------------------------------------------
 // getIfluxvectors (getrusanov.f90:127-193) contracted with the metric
 // vector (sx, sy) of the flux point, as inviscid_flux_interior stores it
@@ -79,9 +92,12 @@ __device__ __forceinline__ void iflux_n(const T *q, T c_h, T sx, T sy, T *Fn, T 
   const T ri = rcp(rho);
   const T u = rhou * ri, v = rhov * ri, w = rhow * ri;
   const T B2 = Bx * Bx + By * By + Bz * Bz;
-  const T pr = T(GAM - 1.0) *
-               (rhoE - T(0.5) * (rhou * u + rhov * v + rhow * w) - T(0.5) * B2 -
-                T(0.5) * psi * psi);
+  T pr;
+  if constexpr (sizeof(T) == 4)
+    pr = pressure_mx(rho, rhou, rhov, rhow, rhoE, Bx, By, Bz, psi);
+  else
+    pr = T(GAM - 1.0) *
+         (rhoE - T(0.5) * (rhou * u + rhov * v + rhow * w) - T(0.5) * B2 - T(0.5) * psi * psi);
   const T pt = pr + T(0.5) * B2;
   const T ub = u * Bx + v * By + w * Bz;
   const T ef = rhoE - T(0.5) * psi * psi + pt;
@@ -115,10 +131,16 @@ __device__ __forceinline__ void rusanov(const T *QL, const T *QR, T nx, T ny,
   const T ur = rir * QR[1], vr = rir * QR[2], wr = rir * QR[3];
   const T B2l = QL[5] * QL[5] + QL[6] * QL[6] + QL[7] * QL[7];
   const T B2r = QR[5] * QR[5] + QR[6] * QR[6] + QR[7] * QR[7];
-  const T pgl = T(GAM - 1.0) * (QL[4] - T(0.5) * QL[0] * (ul * ul + vl * vl + wl * wl) -
-                                B2l * T(0.5) - T(0.5) * QL[8] * QL[8]);
-  const T pgr = T(GAM - 1.0) * (QR[4] - T(0.5) * QR[0] * (ur * ur + vr * vr + wr * wr) -
-                                B2r * T(0.5) - T(0.5) * QR[8] * QR[8]);
+  T pgl, pgr;
+  if constexpr (sizeof(T) == 4) {
+    pgl = pressure_mx(QL[0], QL[1], QL[2], QL[3], QL[4], QL[5], QL[6], QL[7], QL[8]);
+    pgr = pressure_mx(QR[0], QR[1], QR[2], QR[3], QR[4], QR[5], QR[6], QR[7], QR[8]);
+  } else {
+    pgl = T(GAM - 1.0) * (QL[4] - T(0.5) * QL[0] * (ul * ul + vl * vl + wl * wl) -
+                          B2l * T(0.5) - T(0.5) * QL[8] * QL[8]);
+    pgr = T(GAM - 1.0) * (QR[4] - T(0.5) * QR[0] * (ur * ur + vr * vr + wr * wr) -
+                          B2r * T(0.5) - T(0.5) * QR[8] * QR[8]);
+  }
   const T ptl = pgl + T(0.5) * B2l, ptr = pgr + T(0.5) * B2r;
   const T Vnl = ul * ux + vl * uy, Vnr = ur * ux + vr * uy;
   const T Bnl = QL[5] * ux + QL[6] * uy, Bnr = QR[5] * ux + QR[6] * uy;

@@ -21,6 +21,7 @@ struct HostTables {
   std::vector<int> ft;       // [tile][k][{neighbour base, info}][ci]
   std::vector<double> met;   // [tile][metric slot][ci]
   std::vector<double> av;    // hlow[isl][is], then hhigh[is][isl]
+  std::vector<double> metw;  // viscous path: other-family metrics at line ends
 };
 
 namespace hostdetail {
@@ -267,6 +268,26 @@ inline int build_host_tables(const sd2d_mesh *m, const sd2d_params *p, HostTable
       for (int is = 0; is < N; ++is)
         put_tile<double>(t.met, NM, CPW, cell, 4 * N * (N + 1) + js + N * is,
                          1.0 / m->jac[is + N * (js + (size_t)N * cell)]);
+  }
+  // viscous path: S1(2,1:2,rfp,js) and S2(1,1:2,is,rfp) at the two ends of
+  // every line, used by the adiabatic-wall gradient ghost (bcflux.f90:547-571)
+  if (p->vismode == 1 && (p->mu != 0.0 || p->eta0 != 0.0)) {
+    const int NW = 8 * N;
+    t.metw.assign((size_t)nt * NW * CPW, 0.0);
+    for (int cell = 0; cell < nc; ++cell)
+      for (int end = 0; end < 2; ++end) {
+        const int rfp = end ? N : 0;
+        for (int line = 0; line < N; ++line) {
+          const size_t b1 = 2 * (2 * (rfp + (size_t)(N + 1) * (line + (size_t)N * cell)));
+          const size_t b2 = 2 * (2 * (line + (size_t)N * (rfp + (size_t)(N + 1) * cell)));
+          for (int comp = 0; comp < 2; ++comp) {
+            put_tile<double>(t.metw, NW, CPW, cell, 0 * 4 * N + end * 2 * N + comp * N + line,
+                             m->s1[b1 + 1 + 2 * comp]);
+            put_tile<double>(t.metw, NW, CPW, cell, 1 * 4 * N + end * 2 * N + comp * N + line,
+                             m->s2[b2 + 0 + 2 * comp]);
+          }
+        }
+      }
   }
   // AV projections (AV.f90:18-63): hlow[isl][is], hhigh[is][isl]
   t.av.assign((size_t)2 * N * (N - 1), 0.0);

@@ -53,14 +53,17 @@ struct StageArgs {
   double *resid_f;          // resid(9,N,N,NCELL) Fortran layout, or null
   double *nabla_f;          // nablaQs(9,2,N,N,NCELL) Fortran layout, or null
   const double *avtab;      // hlow[isl][is] then hhigh[is][isl] (AV.f90:18-63)
+  const double *visres;     // viscous divergence dFv + dGv (tile layout), M_VIS
   int ncell, ntile;
 };
 
 // MODE bits.  The RK registers read pointwise (none, one, or the four of
 // FiveStageRk's last stage) and the residual store are part of the mode so
 // that the stage update is straight-line code.
+// M_VIS adds the viscous divergence of the residual evaluation (formed by
+// the kernels of sd2d_visc.hpp) before the Jacobian scaling.
 enum { M_RK = 1, M_AV = 2, M_NORMS = 4, M_COMPAT = 8, M_AUX1 = 16, M_AUX4 = 32,
-       M_RSTORE = 64 };
+       M_RSTORE = 64, M_VIS = 128 };
 
 // XCD-aware block order: consecutive positions of the tile launch order go
 // to the same XCD (and its L2), so the neighbour lines a tile reads were
@@ -258,7 +261,7 @@ __global__ __launch_bounds__(STAGE_THREADS, 2) void stage_kernel(StageArgs a, Op
   constexpr bool AV = (MODE & M_AV) != 0, NORMS = (MODE & M_NORMS) != 0;
   constexpr bool COMPAT = (MODE & M_COMPAT) != 0, RK = (MODE & M_RK) != 0;
   constexpr int NAUX = (MODE & M_AUX4) ? 4 : ((MODE & M_AUX1) ? 1 : 0);
-  constexpr bool RSTORE = (MODE & M_RSTORE) != 0;
+  constexpr bool RSTORE = (MODE & M_RSTORE) != 0, VIS = (MODE & M_VIS) != 0;
   static_assert(NT <= STAGE_THREADS, "one thread per grid line");
   extern __shared__ double lds[];
   const int tid = threadIdx.x;
@@ -493,8 +496,12 @@ __global__ __launch_bounds__(STAGE_THREADS, 2) void stage_kernel(StageArgs a, Op
 #pragma unroll
       for (int v = 0; v < 9; ++v) {
         q[v] = lds[LD::Q0 + (v * N2 + p) * T + ci];
-        // resid = -(dF/dxi + dG/deta) / J   (compresid.f90:41,56)
-        R[v] = -D[s][v] * ji;
+        // resid = -(dF/dxi + dG/deta) / J   (compresid.f90:41,56), with the
+        // viscous divergence added before the scaling (compresid.f90:43-46)
+        if (VIS)
+          R[v] = (a.visres[(size_t)tile * G::QT + ci + (size_t)(v * N2 + p) * T] - D[s][v]) * ji;
+        else
+          R[v] = -D[s][v] * ji;
       }
       const double divB = db[s] * ji;
       const double ri = 1.0 / q[0];

@@ -6,9 +6,11 @@
 //   FP32   interpolation to the flux points, the interior fluxes, the
 //          Rusanov face fluxes and boundary ghosts, the line derivatives
 //          and the divB / Jz weights (operators and metrics in FP32);
-//   FP64   the state in HBM, the Jacobian scaling of the residual, the
-//          Powell and GLM sources, the RK stage combination, the norms and
-//          the AV sensor.
+//   FP64   the state in HBM, the pressure recovery inside the fluxes
+//          (E minus kinetic minus magnetic energy, formed from the FP32
+//          traces and rounded after the cancellation), the Jacobian scaling
+//          of the residual, the Powell and GLM sources, the RK stage
+//          combination, the norms and the AV sensor.
 // The tile's state is staged in LDS as FP32 for the traces; the points step
 // reads the FP64 state from memory (one point ahead of its use), and the
 // FP32 staging area is reused for the exchange of the eta-line derivatives.

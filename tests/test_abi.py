@@ -109,10 +109,11 @@ def test_bad_k_stage():
     assert rc == abi.SD2D_EINVAL and "k_stage" in msg
 
 
-def test_viscous_rejected_on_device():
-    """mu/eta0 != 0 with vismode=1 is the viscous path, which the device
-    does not implement: refused rather than silently dropped."""
-    rc, _, msg = _create(_small_case(vismode=1, mu=0.01))
+def test_viscous_mixed_rejected():
+    """The viscous/resistive path (vismode=1, mu/eta0 != 0) runs in FP64
+    only: asking for it in mixed precision is refused rather than silently
+    run in FP64."""
+    rc, _, msg = _create(_small_case(vismode=1, mu=0.01), precision=abi.SD2D_MIXED)
     assert rc == abi.SD2D_EUNSUPPORTED and "viscous" in msg
 
 
