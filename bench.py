@@ -104,6 +104,8 @@ def main() -> int:
                          "time, reporting the mixed form's L2 difference")
     ap.add_argument("--cpu-steps", type=int, default=1,
                     help="steps of the CPU baseline sample (0 = skip)")
+    ap.add_argument("--tile-shape", default="strips", choices=["strips", "blocks"],
+                    help="cells of one workgroup: mesh-order strips (default) or compact blocks")
     ap.add_argument("--no-mixed", action="store_true",
                     help="skip the mixed-precision comparison")
     args = ap.parse_args()
@@ -113,7 +115,7 @@ def main() -> int:
                          "run with --gpus 1\n")
         return 2
 
-    from sddc_amd import cases
+    from sddc_amd import abi, cases
     from sddc_amd.solver import Sd2d
 
     t = time.perf_counter()
@@ -125,7 +127,8 @@ def main() -> int:
         return 0
     N = case.N
     ndof = cases.dof(case)
-    g = Sd2d(case)
+    shape = abi.SD2D_TILE_STRIPS if args.tile_shape == "strips" else abi.SD2D_TILE_BLOCKS
+    g = Sd2d(case, tile_shape=shape)
     g.set_state(Q0, [0.0, 0.0, 0.0, 0.0])
     g.step(args.warmup, 10)
     print(f"[bench] warmup {args.warmup} steps done", file=sys.stderr, flush=True)
@@ -164,7 +167,7 @@ def main() -> int:
         "data": "synthetic (seeded free stream with 1e-3 density perturbation)",
         "config": {"workload": desc, "ncell": case.ncell, "N": N, "dof": ndof,
                    "dt": case.params.dt, "rk": "RK_S5P3", "av": True,
-                   "norm_every": 10},
+                   "norm_every": 10, "tile_shape": args.tile_shape},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic_from_profiles("stage_kernel_N5_rk"),
@@ -180,7 +183,7 @@ def main() -> int:
         # the relative L2 difference from the FP64 state after the same steps
         import numpy as np
         from sddc_amd import abi
-        gm = Sd2d(case, precision=abi.SD2D_MIXED)
+        gm = Sd2d(case, precision=abi.SD2D_MIXED, tile_shape=shape)
         gm.set_state(Q0, [0.0, 0.0, 0.0, 0.0])
         gm.step(args.warmup, 10)
         tm0 = time.perf_counter()

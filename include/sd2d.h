@@ -48,6 +48,18 @@ typedef struct sd2d_ctx sd2d_ctx;
  * sd2d_compresid always evaluates in FP64. */
 enum { SD2D_FP64 = 0, SD2D_MIXED = 1 };
 
+/* Tile shape: the device processes cells in tiles of 12/16/20 (N = 5/4/3),
+ * one workgroup per tile.  SD2D_TILE_STRIPS takes consecutive cells of the
+ * mesh numbering; SD2D_TILE_BLOCKS groups compact patches of cells
+ * (recursive bisection of the cell adjacency graph), so fewer neighbour
+ * traces come from other tiles, but those that do are scattered over more
+ * tiles.  SD2D_TILE_DEFAULT is SD2D_TILE_STRIPS (the faster of the two on
+ * structured O-grid and periodic meshes numbered row by row).  Results do
+ * not depend on the shape (every face is evaluated with the same operands
+ * either way); only the speed does.  No counterpart in the reference,
+ * whose cells are processed in file order. */
+enum { SD2D_TILE_DEFAULT = 0, SD2D_TILE_STRIPS = 1, SD2D_TILE_BLOCKS = 2 };
+
 /*
  * Mesh, operators and metrics exactly as module setup2d holds them after
  * INIT_SETUP (initsetup.f90:1-169) on one rank.
@@ -96,6 +108,7 @@ typedef struct sd2d_params {
   int32_t k_stage;      /* 3: RK_S5P3, 5: FiveStageRk (iterations.f90:22)  */
   double dt;            /* fixed time step                                 */
   int32_t precision;    /* SD2D_FP64 or SD2D_MIXED                         */
+  int32_t tile_shape;   /* SD2D_TILE_*: which cells share a workgroup      */
 } sd2d_params;
 
 /* Running maxima of the GLM wave speed (F1 in the path description):

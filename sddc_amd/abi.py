@@ -16,6 +16,7 @@ from . import mesh as M
 SD2D_OK, SD2D_EINVAL, SD2D_EHIP, SD2D_ENOMEM, SD2D_ENONFINITE, \
     SD2D_EUNSUPPORTED = range(6)
 SD2D_FP64, SD2D_MIXED = 0, 1
+SD2D_TILE_DEFAULT, SD2D_TILE_STRIPS, SD2D_TILE_BLOCKS = 0, 1, 2
 
 _pi = C.POINTER(C.c_int32)
 _pd = C.POINTER(C.c_double)
@@ -44,7 +45,8 @@ class Sd2dParams(C.Structure):
                 ("bxinf", C.c_double), ("byinf", C.c_double),
                 ("dxcycl", C.c_double), ("dycycl", C.c_double),
                 ("ncellglob", C.c_int32), ("k_stage", C.c_int32),
-                ("dt", C.c_double), ("precision", C.c_int32)]
+                ("dt", C.c_double), ("precision", C.c_int32),
+                ("tile_shape", C.c_int32)]
 
 
 class Sd2dMaxima(C.Structure):
@@ -102,7 +104,8 @@ def mesh_struct(case: M.Case):
 
 
 def params_struct(p: M.RunParams, ncellglob: int,
-                  precision: int = SD2D_FP64) -> Sd2dParams:
+                  precision: int = SD2D_FP64,
+                  tile_shape: int = SD2D_TILE_DEFAULT) -> Sd2dParams:
     s = Sd2dParams()
     s.vismode = p.vismode; s.artiv = p.artiv
     s.mu = p.mu; s.eta0 = p.eta0; s.alpha = p.alpha
@@ -111,4 +114,5 @@ def params_struct(p: M.RunParams, ncellglob: int,
     s.dxcycl = p.dxcycl; s.dycycl = p.dycycl
     s.ncellglob = ncellglob; s.k_stage = p.k_stage; s.dt = p.dt
     s.precision = precision
+    s.tile_shape = tile_shape
     return s

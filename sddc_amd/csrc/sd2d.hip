@@ -40,6 +40,7 @@ struct sd2d_ctx {
   double *met = nullptr;
   int *ftab = nullptr;
   int *torder = nullptr;
+  int *c2s = nullptr, *s2c = nullptr;          // cell <-> slot maps of the tile shape
   double *avtab = nullptr;
   double *gmax = nullptr;                       // eigvmax_g, umax_g
   unsigned long long *slots = nullptr;
@@ -84,11 +85,11 @@ static int fail(sd2d_ctx *c, int code, const std::string &m) {
 // ---------------------------------------------------------------------------
 // small device kernels
 // ---------------------------------------------------------------------------
-// Fortran Q(9,N,N,NCELL) -> tile-major SoA (padded cells get free-stream-free
-// zeros; they are never read by a valid lane).
+// Fortran Q(9,N,N,NCELL) -> tile-major SoA through the slot -> cell map
+// (the padding of the last tile gets zeros; no valid lane reads it).
 template <int N>
-__global__ void k_f2t(const double *__restrict__ f, double *__restrict__ t, int ncell,
-                      int ntile) {
+__global__ void k_f2t(const double *__restrict__ f, double *__restrict__ t,
+                      const int *__restrict__ s2c, int ntile) {
   constexpr int CPW = Geo<N>::CPW, N2 = Geo<N>::N2;
   const size_t tot = (size_t)ntile * CPW * 9 * N2;
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < tot;
@@ -98,13 +99,14 @@ __global__ void k_f2t(const double *__restrict__ f, double *__restrict__ t, int 
     const int vp = int(r % (9 * N2));
     const size_t tile = r / (9 * N2);
     const int v = vp / N2, p = vp % N2;
-    const size_t cell = tile * CPW + ci;
-    t[i] = cell < (size_t)ncell ? f[v + 9 * (p + (size_t)N2 * cell)] : 0.0;
+    const int cell = s2c[tile * CPW + ci];
+    t[i] = cell >= 0 ? f[v + 9 * (p + (size_t)N2 * cell)] : 0.0;
   }
 }
 
 template <int N>
-__global__ void k_t2f(const double *__restrict__ t, double *__restrict__ f, int ncell) {
+__global__ void k_t2f(const double *__restrict__ t, double *__restrict__ f,
+                      const int *__restrict__ c2s, int ncell) {
   constexpr int CPW = Geo<N>::CPW, N2 = Geo<N>::N2;
   const size_t tot = (size_t)ncell * 9 * N2;
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < tot;
@@ -112,8 +114,8 @@ __global__ void k_t2f(const double *__restrict__ t, double *__restrict__ f, int 
     const int v = int(i % 9);
     const size_t r = i / 9;
     const int p = int(r % N2);
-    const size_t cell = r / N2;
-    const size_t tile = cell / CPW, ci = cell % CPW;
+    const size_t slot = c2s[r / N2];
+    const size_t tile = slot / CPW, ci = slot % CPW;
     f[i] = t[(tile * 9 * N2 + v * N2 + p) * CPW + ci];
   }
 }
@@ -300,6 +302,7 @@ static StageArgs base_args(sd2d_ctx *c) {
   a.ncell = c->ncell;
   a.ntile = c->ntile;
   a.sm_ind = c->sm_ind;
+  a.s2c = c->s2c;
   return a;
 }
 
@@ -379,6 +382,8 @@ int sd2d_create(const sd2d_mesh *m, const sd2d_params *p, sd2d_ctx **out) {
   }
   CK(hipMalloc(&c->ftab, ft.size() * sizeof(int)));
   CK(hipMalloc(&c->torder, tb.torder.size() * sizeof(int)));
+  CK(hipMalloc(&c->c2s, tb.c2s.size() * sizeof(int)));
+  CK(hipMalloc(&c->s2c, tb.s2c.size() * sizeof(int)));
   CK(hipMalloc(&c->avtab, av.size() * sizeof(double)));
   CK(hipMalloc(&c->gmax, 2 * sizeof(double)));
   CK(hipMalloc(&c->slots, 2 * NSLOT * sizeof(unsigned long long)));
@@ -402,6 +407,10 @@ int sd2d_create(const sd2d_mesh *m, const sd2d_params *p, sd2d_ctx **out) {
   CK(hipMemcpyAsync(c->ftab, ft.data(), ft.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
   CK(hipMemcpyAsync(c->torder, tb.torder.data(), tb.torder.size() * sizeof(int),
                     hipMemcpyHostToDevice, c->stream));
+  CK(hipMemcpyAsync(c->c2s, tb.c2s.data(), tb.c2s.size() * sizeof(int), hipMemcpyHostToDevice,
+                    c->stream));
+  CK(hipMemcpyAsync(c->s2c, tb.s2c.data(), tb.s2c.size() * sizeof(int), hipMemcpyHostToDevice,
+                    c->stream));
   CK(hipMemcpyAsync(c->avtab, av.data(), av.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
   for (int i = 0; i < 3; ++i) CK(hipMemsetAsync(c->q[i], 0, qn * sizeof(double), c->stream));
   CK(hipMemsetAsync(c->sm_ind, 0, 2 * (size_t)nc * sizeof(double), c->stream));
@@ -420,7 +429,8 @@ void sd2d_destroy(sd2d_ctx *c) {
   for (double *b : c->q) if (b) (void)hipFree(b);
   for (double *b : c->fsr) if (b) (void)hipFree(b);
   void *ptrs[] = {c->met, c->metf, c->ftab, c->torder, c->avtab, c->gmax, c->slots, c->sm_ind, c->normbuf,
-                  c->normout, c->stagef, c->nablaf, c->gq, c->dei, c->visres, c->metw};
+                  c->normout, c->stagef, c->nablaf, c->gq, c->dei, c->visres, c->metw,
+                  c->c2s, c->s2c};
   for (void *b : ptrs) if (b) (void)hipFree(b);
   for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -430,9 +440,9 @@ void sd2d_destroy(sd2d_ctx *c) {
 static int f2t(sd2d_ctx *c, const double *dev_f, double *dev_t) {
   const int blocks = 1024;
   switch (c->N) {
-    case 3: hipLaunchKernelGGL(k_f2t<3>, dim3(blocks), dim3(256), 0, c->stream, dev_f, dev_t, c->ncell, c->ntile); break;
-    case 4: hipLaunchKernelGGL(k_f2t<4>, dim3(blocks), dim3(256), 0, c->stream, dev_f, dev_t, c->ncell, c->ntile); break;
-    default: hipLaunchKernelGGL(k_f2t<5>, dim3(blocks), dim3(256), 0, c->stream, dev_f, dev_t, c->ncell, c->ntile); break;
+    case 3: hipLaunchKernelGGL(k_f2t<3>, dim3(blocks), dim3(256), 0, c->stream, dev_f, dev_t, c->s2c, c->ntile); break;
+    case 4: hipLaunchKernelGGL(k_f2t<4>, dim3(blocks), dim3(256), 0, c->stream, dev_f, dev_t, c->s2c, c->ntile); break;
+    default: hipLaunchKernelGGL(k_f2t<5>, dim3(blocks), dim3(256), 0, c->stream, dev_f, dev_t, c->s2c, c->ntile); break;
   }
   HIPCHK(c, hipGetLastError());
   return SD2D_OK;
@@ -441,9 +451,9 @@ static int f2t(sd2d_ctx *c, const double *dev_f, double *dev_t) {
 static int t2f(sd2d_ctx *c, const double *dev_t, double *dev_f) {
   const int blocks = 1024;
   switch (c->N) {
-    case 3: hipLaunchKernelGGL(k_t2f<3>, dim3(blocks), dim3(256), 0, c->stream, dev_t, dev_f, c->ncell); break;
-    case 4: hipLaunchKernelGGL(k_t2f<4>, dim3(blocks), dim3(256), 0, c->stream, dev_t, dev_f, c->ncell); break;
-    default: hipLaunchKernelGGL(k_t2f<5>, dim3(blocks), dim3(256), 0, c->stream, dev_t, dev_f, c->ncell); break;
+    case 3: hipLaunchKernelGGL(k_t2f<3>, dim3(blocks), dim3(256), 0, c->stream, dev_t, dev_f, c->c2s, c->ncell); break;
+    case 4: hipLaunchKernelGGL(k_t2f<4>, dim3(blocks), dim3(256), 0, c->stream, dev_t, dev_f, c->c2s, c->ncell); break;
+    default: hipLaunchKernelGGL(k_t2f<5>, dim3(blocks), dim3(256), 0, c->stream, dev_t, dev_f, c->c2s, c->ncell); break;
   }
   HIPCHK(c, hipGetLastError());
   return SD2D_OK;

@@ -15,6 +15,8 @@ namespace sd2d {
 
 struct HostTables {
   int N = 0, ncell = 0, cpw = 0, ntile = 0;
+  std::vector<int> c2s;      // cell -> slot (tile*cpw + position in the tile)
+  std::vector<int> s2c;      // slot -> cell, -1 for the padding of the last tile
   Ops ops{};
   Phys ph{};
   std::vector<int> torder;   // launch order of the tiles (see tile_order)
@@ -67,8 +69,8 @@ inline double lagrange(const double *x, int n, int i, double t) {
 
 // Per-cell face table and the check that the left cell's normal equals +-
 // the right cell's own metric vector bit for bit at every face point.
-inline int build_face_table(const sd2d_mesh *m, int CPW, std::vector<int> &nb,
-                            std::vector<int> &info, std::string &err) {
+inline int build_face_table(const sd2d_mesh *m, int CPW, const std::vector<int> &c2s,
+                            std::vector<int> &nb, std::vector<int> &info, std::string &err) {
   using hostdetail::HostMesh;
   const int N = m->N, nc = m->ncell, N2 = N * N;
   HostMesh hm{N, nc, m->nface, m};
@@ -76,7 +78,7 @@ inline int build_face_table(const sd2d_mesh *m, int CPW, std::vector<int> &nb,
   info.assign((size_t)nc * 4, -1);
   auto chk_face = [&](int f) { return f >= 1 && f <= m->nface; };
   auto base_of = [&](int cell) {
-    const size_t tile = cell / CPW, ci = cell % CPW;
+    const size_t slot = c2s[cell], tile = slot / CPW, ci = slot % CPW;
     return (long long)(tile * 9 * N2 * CPW + ci);
   };
   auto bad = [&](int code, const char *msg) { err = msg; return code; };
@@ -146,38 +148,17 @@ inline int build_face_table(const sd2d_mesh *m, int CPW, std::vector<int> &nb,
   return SD2D_OK;
 }
 
-// Launch order of the tiles: recursive bisection of the tile adjacency
-// graph (two tiles are adjacent when a face joins their cells).  Each part
-// is split at the median of the breadth-first distance from a
-// pseudo-peripheral tile of the part, so that every part, down to groups
-// of a few tiles, is a compact patch of the mesh and consecutive tiles in
-// the order are neighbours.  Tiles that are processed close together in
-// time then find each other's states in the L2 cache of their XCD,
-// whatever the cell numbering of the mesh.
-inline std::vector<int> tile_order(int nt, int CPW, int tile_doubles,
-                                   const std::vector<int> &nb, const std::vector<int> &info) {
-  // adjacency (CSR, deduplicated)
-  std::vector<std::vector<int>> adj(nt);
-  const int nc = (int)(nb.size() / 4);
-  for (int cell = 0; cell < nc; ++cell)
-    for (int k = 0; k < 4; ++k) {
-      const size_t i = 4 * (size_t)cell + k;
-      if (((info[i] >> 3) & 7) != FT_INTERIOR) continue;
-      const int t = cell / CPW, u = nb[i] / tile_doubles;
-      if (u != t) adj[t].push_back(u);
-    }
-  std::vector<int> xadj(nt + 1, 0), ladj;
-  for (int t = 0; t < nt; ++t) {
-    std::sort(adj[t].begin(), adj[t].end());
-    adj[t].erase(std::unique(adj[t].begin(), adj[t].end()), adj[t].end());
-    xadj[t + 1] = xadj[t] + (int)adj[t].size();
-    ladj.insert(ladj.end(), adj[t].begin(), adj[t].end());
-  }
-  std::vector<int> ord(nt), part(nt, -1), dist(nt, -1), queue;
-  for (int t = 0; t < nt; ++t) ord[t] = t;
-  queue.reserve(nt);
-  // breadth-first distances inside part `pid` from `src`; returns the last
-  // tile reached (a farthest one)
+// Breadth-first recursive bisection of a graph given in CSR form.  A part
+// [b, e) of the vertex order holding more than one grain of vertices is
+// sorted by breadth-first distance from a pseudo-peripheral vertex and
+// split after floor(grains/2) grains (the part's last grain takes any
+// remainder), down to single grains; parts at every level are compact and
+// consecutive vertices of the result lie close together in the graph.
+inline std::vector<int> bisection_order(int n, int grain, const std::vector<int> &xadj,
+                                        const std::vector<int> &ladj) {
+  std::vector<int> ord(n), part(n, -1), dist(n, -1), queue;
+  for (int t = 0; t < n; ++t) ord[t] = t;
+  queue.reserve(n);
   auto bfs = [&](int pid, int src, int b, int e) {
     for (int i = b; i < e; ++i) dist[ord[i]] = -1;
     queue.clear();
@@ -192,16 +173,16 @@ inline std::vector<int> tile_order(int nt, int CPW, int tile_doubles,
     }
     return queue.back();
   };
-  constexpr int LEAF = 4;
-  std::vector<std::pair<int, int>> stack{{0, nt}};
+  std::vector<std::pair<int, int>> stack{{0, n}};
   int pid = 0;
   while (!stack.empty()) {
     const auto [b, e] = stack.back();
     stack.pop_back();
-    if (e - b <= LEAF) continue;
+    const int ng = (e - b + grain - 1) / grain;   // grains in the part
+    if (ng <= 1) continue;
     ++pid;
     for (int i = b; i < e; ++i) part[ord[i]] = pid;
-    // pseudo-peripheral start: the farthest tile from the lowest-numbered one
+    // pseudo-peripheral start: the farthest vertex from the lowest-numbered one
     const int first = *std::min_element(ord.begin() + b, ord.begin() + e);
     const int p = bfs(pid, first, b, e);
     bfs(pid, p, b, e);
@@ -210,11 +191,82 @@ inline std::vector<int> tile_order(int nt, int CPW, int tile_doubles,
       const long long dy = dist[y] < 0 ? (1LL << 40) : dist[y];
       return dx != dy ? dx < dy : x < y;
     });
-    const int m = b + (e - b) / 2;
+    const int m = b + (ng / 2) * grain;
     stack.push_back({m, e});     // second half after the first
     stack.push_back({b, m});
   }
   return ord;
+}
+
+// Cell -> slot map of a tile shape.  Strips: slot = cell.  Blocks: the
+// cells in the order of a recursive bisection of the cell adjacency graph
+// (interior faces and periodic pairs) with one tile as the grain, so every
+// tile is a compact patch of cells; the padding stays at the end.
+inline void tile_slots(const sd2d_mesh *m, int CPW, int shape, std::vector<int> &c2s,
+                       std::vector<int> &s2c) {
+  const int nc = m->ncell, nt = (nc + CPW - 1) / CPW;
+  std::vector<int> order(nc);
+  for (int c = 0; c < nc; ++c) order[c] = c;
+  if (shape == SD2D_TILE_BLOCKS) {
+    std::vector<std::vector<int>> adj(nc);
+    auto link = [&](int f1, int k1, int f2, int k2) {
+      if (f1 < 1 || f1 > m->nface || f2 < 1 || f2 > m->nface) return;
+      const int a = m->if2c[(f1 - 1) + (size_t)m->nface * k1] - 1;
+      const int b = m->if2c[(f2 - 1) + (size_t)m->nface * k2] - 1;
+      if (a >= 0 && a < nc && b >= 0 && b < nc && a != b) {
+        adj[a].push_back(b);
+        adj[b].push_back(a);
+      }
+    };
+    for (int i = 0; i < m->ninter; ++i) link(m->ibfinter[i], 0, m->ibfinter[i], 1);
+    for (int i = 0; i < m->npair; ++i) link(m->cyc2fpair[i], 0, m->cyc2fpair[i + m->npair], 0);
+    std::vector<int> xadj(nc + 1, 0), ladj;
+    for (int c = 0; c < nc; ++c) {
+      std::sort(adj[c].begin(), adj[c].end());
+      adj[c].erase(std::unique(adj[c].begin(), adj[c].end()), adj[c].end());
+      xadj[c + 1] = xadj[c] + (int)adj[c].size();
+      ladj.insert(ladj.end(), adj[c].begin(), adj[c].end());
+    }
+    order = bisection_order(nc, CPW, xadj, ladj);
+  }
+  c2s.assign(nc, 0);
+  s2c.assign((size_t)nt * CPW, -1);
+  for (int sl = 0; sl < nc; ++sl) {
+    c2s[order[sl]] = sl;
+    s2c[sl] = order[sl];
+  }
+}
+
+// Launch order of the tiles: recursive bisection of the tile adjacency
+// graph (two tiles are adjacent when a face joins their cells).  Each part
+// is split at the median of the breadth-first distance from a
+// pseudo-peripheral tile of the part, so that every part, down to groups
+// of a few tiles, is a compact patch of the mesh and consecutive tiles in
+// the order are neighbours.  Tiles that are processed close together in
+// time then find each other's states in the L2 cache of their XCD,
+// whatever the cell numbering of the mesh.
+inline std::vector<int> tile_order(int nt, int CPW, int tile_doubles,
+                                   const std::vector<int> &nb, const std::vector<int> &info,
+                                   const std::vector<int> &c2s) {
+  // adjacency (CSR, deduplicated)
+  std::vector<std::vector<int>> adj(nt);
+  const int nc = (int)(nb.size() / 4);
+  for (int cell = 0; cell < nc; ++cell)
+    for (int k = 0; k < 4; ++k) {
+      const size_t i = 4 * (size_t)cell + k;
+      if (((info[i] >> 3) & 7) != FT_INTERIOR) continue;
+      const int t = c2s[cell] / CPW, u = nb[i] / tile_doubles;
+      if (u != t) adj[t].push_back(u);
+    }
+  std::vector<int> xadj(nt + 1, 0), ladj;
+  for (int t = 0; t < nt; ++t) {
+    std::sort(adj[t].begin(), adj[t].end());
+    adj[t].erase(std::unique(adj[t].begin(), adj[t].end()), adj[t].end());
+    xadj[t + 1] = xadj[t] + (int)adj[t].size();
+    ladj.insert(ladj.end(), adj[t].begin(), adj[t].end());
+  }
+  constexpr int LEAF = 4;
+  return bisection_order(nt, LEAF, xadj, ladj);
 }
 
 // Everything the kernels read besides the state.
@@ -232,21 +284,27 @@ inline int build_host_tables(const sd2d_mesh *m, const sd2d_params *p, HostTable
   t.ph.vinf = p->vinf; t.ph.bxinf = p->bxinf; t.ph.byinf = p->byinf;
   t.ph.vismode = p->vismode;
 
+  if (p->tile_shape < SD2D_TILE_DEFAULT || p->tile_shape > SD2D_TILE_BLOCKS) {
+    err = "tile_shape must be SD2D_TILE_DEFAULT, SD2D_TILE_STRIPS or SD2D_TILE_BLOCKS";
+    return SD2D_EINVAL;
+  }
+  tile_slots(m, CPW, p->tile_shape == SD2D_TILE_BLOCKS ? SD2D_TILE_BLOCKS : SD2D_TILE_STRIPS,
+             t.c2s, t.s2c);
   std::vector<int> nb, info;
-  const int r = build_face_table(m, CPW, nb, info, err);
+  const int r = build_face_table(m, CPW, t.c2s, nb, info, err);
   if (r) return r;
   const int nt = t.ntile;
-  t.torder = tile_order(nt, CPW, 9 * N2 * CPW, nb, info);
+  t.torder = tile_order(nt, CPW, 9 * N2 * CPW, nb, info, t.c2s);
   t.ft.assign((size_t)nt * 8 * CPW, 0);
   for (int cell = 0; cell < nc; ++cell)
     for (int k = 0; k < 4; ++k) {
-      put_tile<int>(t.ft, 8, CPW, cell, 2 * k, nb[4 * (size_t)cell + k]);
-      put_tile<int>(t.ft, 8, CPW, cell, 2 * k + 1, info[4 * (size_t)cell + k]);
+      put_tile<int>(t.ft, 8, CPW, t.c2s[cell], 2 * k, nb[4 * (size_t)cell + k]);
+      put_tile<int>(t.ft, 8, CPW, t.c2s[cell], 2 * k + 1, info[4 * (size_t)cell + k]);
     }
-  // padded cells: FREE boundary so that nothing is dereferenced for them
-  for (int cell = nc; cell < nt * CPW; ++cell)
+  // padded slots: FREE boundary so that nothing is dereferenced for them
+  for (int sl = nc; sl < nt * CPW; ++sl)
     for (int k = 0; k < 4; ++k)
-      put_tile<int>(t.ft, 8, CPW, cell, 2 * k + 1, finfo_pack(0, 1, FT_FREE, 0));
+      put_tile<int>(t.ft, 8, CPW, sl, 2 * k + 1, finfo_pack(0, 1, FT_FREE, 0));
 
   // metrics: S1(1,1:2,rfp,js), S2(2,1:2,is,rfp), 1/Jac(is,js)
   const int NM = 4 * N * (N + 1) + N2;
@@ -255,18 +313,18 @@ inline int build_host_tables(const sd2d_mesh *m, const sd2d_params *p, HostTable
     for (int js = 0; js < N; ++js)
       for (int rfp = 0; rfp <= N; ++rfp) {
         const size_t b = 2 * (2 * (rfp + (size_t)(N + 1) * (js + (size_t)N * cell)));
-        put_tile<double>(t.met, NM, CPW, cell, 0 + rfp + (N + 1) * js, m->s1[b + 0]);
-        put_tile<double>(t.met, NM, CPW, cell, N * (N + 1) + rfp + (N + 1) * js, m->s1[b + 2]);
+        put_tile<double>(t.met, NM, CPW, t.c2s[cell], 0 + rfp + (N + 1) * js, m->s1[b + 0]);
+        put_tile<double>(t.met, NM, CPW, t.c2s[cell], N * (N + 1) + rfp + (N + 1) * js, m->s1[b + 2]);
       }
     for (int rfp = 0; rfp <= N; ++rfp)
       for (int is = 0; is < N; ++is) {
         const size_t b = 2 * (2 * (is + (size_t)N * (rfp + (size_t)(N + 1) * cell)));
-        put_tile<double>(t.met, NM, CPW, cell, 2 * N * (N + 1) + rfp + (N + 1) * is, m->s2[b + 1]);
-        put_tile<double>(t.met, NM, CPW, cell, 3 * N * (N + 1) + rfp + (N + 1) * is, m->s2[b + 3]);
+        put_tile<double>(t.met, NM, CPW, t.c2s[cell], 2 * N * (N + 1) + rfp + (N + 1) * is, m->s2[b + 1]);
+        put_tile<double>(t.met, NM, CPW, t.c2s[cell], 3 * N * (N + 1) + rfp + (N + 1) * is, m->s2[b + 3]);
       }
     for (int js = 0; js < N; ++js)
       for (int is = 0; is < N; ++is)
-        put_tile<double>(t.met, NM, CPW, cell, 4 * N * (N + 1) + js + N * is,
+        put_tile<double>(t.met, NM, CPW, t.c2s[cell], 4 * N * (N + 1) + js + N * is,
                          1.0 / m->jac[is + N * (js + (size_t)N * cell)]);
   }
   // viscous path: S1(2,1:2,rfp,js) and S2(1,1:2,is,rfp) at the two ends of
@@ -281,9 +339,9 @@ inline int build_host_tables(const sd2d_mesh *m, const sd2d_params *p, HostTable
           const size_t b1 = 2 * (2 * (rfp + (size_t)(N + 1) * (line + (size_t)N * cell)));
           const size_t b2 = 2 * (2 * (line + (size_t)N * (rfp + (size_t)(N + 1) * cell)));
           for (int comp = 0; comp < 2; ++comp) {
-            put_tile<double>(t.metw, NW, CPW, cell, 0 * 4 * N + end * 2 * N + comp * N + line,
+            put_tile<double>(t.metw, NW, CPW, t.c2s[cell], 0 * 4 * N + end * 2 * N + comp * N + line,
                              m->s1[b1 + 1 + 2 * comp]);
-            put_tile<double>(t.metw, NW, CPW, cell, 1 * 4 * N + end * 2 * N + comp * N + line,
+            put_tile<double>(t.metw, NW, CPW, t.c2s[cell], 1 * 4 * N + end * 2 * N + comp * N + line,
                              m->s2[b2 + 0 + 2 * comp]);
           }
         }

@@ -54,6 +54,7 @@ struct StageArgs {
   double *nabla_f;          // nablaQs(9,2,N,N,NCELL) Fortran layout, or null
   const double *avtab;      // hlow[isl][is] then hhigh[is][isl] (AV.f90:18-63)
   const double *visres;     // viscous divergence dFv + dGv (tile layout), M_VIS
+  const int *s2c;           // slot -> mesh cell, for the cell-indexed outputs
   int ncell, ntile;
 };
 
@@ -473,7 +474,7 @@ __global__ __launch_bounds__(STAGE_THREADS, 2) void stage_kernel(StageArgs a, Op
         const int p = p0 + ps * s;
         const int is = p % N, js = p / N;
         const double ji = mt[(G::JI + js + N * is) * T];
-        double *o = a.nabla_f + v + 9 * 2 * ((size_t)p + (size_t)N2 * cell);
+        double *o = a.nabla_f + v + 9 * 2 * ((size_t)p + (size_t)N2 * a.s2c[cell]);
         if (add) { o[0] += ax * ji; o[9] += ay * ji; }
         else { o[0] = ax * ji; o[9] = ay * ji; }
       }
@@ -526,7 +527,7 @@ __global__ __launch_bounds__(STAGE_THREADS, 2) void stage_kernel(StageArgs a, Op
         ndiv = fma(divB, divB, ndiv);
       }
       if (COMPAT) {
-        const size_t pf = (size_t)p + (size_t)N2 * cell;
+        const size_t pf = (size_t)p + (size_t)N2 * a.s2c[cell];
 #pragma unroll
         for (int v = 0; v < 9; ++v) a.resid_f[v + 9 * pf] = R[v];
       }
@@ -666,8 +667,9 @@ __global__ __launch_bounds__(STAGE_THREADS, 2) void stage_kernel(StageArgs a, Op
         s1 += seb[(0 * N2 + p) * T + c];
         s2 += seb[(1 * N2 + p) * T + c];
       }
-      a.sm_ind[2 * (size_t)(tile * T + c)] = log10(s1);
-      a.sm_ind[2 * (size_t)(tile * T + c) + 1] = log10(s2);
+      const size_t mc = a.s2c[tile * T + c];
+      a.sm_ind[2 * mc] = log10(s1);
+      a.sm_ind[2 * mc + 1] = log10(s2);
     }
   }
 }

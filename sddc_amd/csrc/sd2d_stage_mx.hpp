@@ -378,8 +378,9 @@ void stage_kernel_mx(StageArgs a, OpsF ops,
         s1 += seb[(0 * N2 + p) * T + c];
         s2 += seb[(1 * N2 + p) * T + c];
       }
-      a.sm_ind[2 * (size_t)(tile * T + c)] = log10(s1);
-      a.sm_ind[2 * (size_t)(tile * T + c) + 1] = log10(s2);
+      const size_t mc = a.s2c[tile * T + c];
+      a.sm_ind[2 * mc] = log10(s1);
+      a.sm_ind[2 * mc + 1] = log10(s2);
     }
   }
 }

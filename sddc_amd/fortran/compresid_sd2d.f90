@@ -78,6 +78,7 @@ subroutine compresid_sd2d_create
   p%k_stage = k_stage
   p%dt = dt
   p%precision = SD2D_FP64
+  p%tile_shape = SD2D_TILE_DEFAULT
 
   rc = sd2d_create(m, p, sd2d_ctx)
   if (rc /= SD2D_OK) then

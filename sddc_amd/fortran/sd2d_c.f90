@@ -14,6 +14,8 @@ module sd2d_c
        SD2D_EHIP = 2, SD2D_ENOMEM = 3, SD2D_ENONFINITE = 4, &
        SD2D_EUNSUPPORTED = 5
   integer(c_int32_t), parameter, public :: SD2D_FP64 = 0, SD2D_MIXED = 1
+  integer(c_int32_t), parameter, public :: SD2D_TILE_DEFAULT = 0, &
+       SD2D_TILE_STRIPS = 1, SD2D_TILE_BLOCKS = 2
 
   type, bind(C), public :: sd2d_mesh_t
      integer(c_int32_t) :: N, ncell, nface
@@ -42,7 +44,7 @@ module sd2d_c
      real(c_double) :: dxcycl, dycycl
      integer(c_int32_t) :: ncellglob, k_stage
      real(c_double) :: dt
-     integer(c_int32_t) :: precision
+     integer(c_int32_t) :: precision, tile_shape
   end type sd2d_params_t
 
   type, bind(C), public :: sd2d_maxima_t

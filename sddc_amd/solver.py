@@ -75,12 +75,13 @@ def _ptr(a):
 class Sd2d:
     """One device context for one mesh (one GPU)."""
 
-    def __init__(self, case: M.Case, precision: int = abi.SD2D_FP64):
+    def __init__(self, case: M.Case, precision: int = abi.SD2D_FP64,
+                 tile_shape: int = abi.SD2D_TILE_DEFAULT):
         self.L = load_library()
         self.case = case
         self.N, self.nc = case.N, case.ncell
         ms, self._keep = abi.mesh_struct(case)
-        ps = abi.params_struct(case.params, case.ncell, precision)
+        ps = abi.params_struct(case.params, case.ncell, precision, tile_shape)
         h = C.c_void_p()
         rc = self.L.sd2d_create(C.byref(ms), C.byref(ps), C.byref(h))
         if rc != 0:
