@@ -79,6 +79,79 @@ def mixed_to_time(case, Q0, t_end: float, desc: str) -> dict:
             "all_finite": bool(np.all(np.isfinite(a)))}
 
 
+def run_c4(steps: int, warmup: int, tune_steps: int = 2) -> dict:
+    """The C4 ensemble (64 cylinder runs of uneven size and order) stepped
+    as one sd2d_ens ensemble in mixed precision after the tile-shape
+    autotuner, against the same ensemble in FP64 and against the 64 runs
+    stepped one after another through sd2d_step (mixed)."""
+    import numpy as np
+    from sddc_amd import abi, cases
+    from sddc_amd.solver import Sd2d, Sd2dEnsemble
+    t = time.perf_counter()
+    mem = cases.c4_ensemble()
+    ndof = sum(cases.dof(c) for c, _, _ in mem)
+    print(f"[bench] c4: {len(mem)} members, {ndof / 1e6:.1f} M DOF, built in "
+          f"{time.perf_counter() - t:.1f}s", file=sys.stderr, flush=True)
+    res, states = {}, {}
+    for prec in (abi.SD2D_MIXED, abi.SD2D_FP64):
+        ens = Sd2dEnsemble([c for c, _, _ in mem], precision=prec)
+        for i, (c, Q0, _) in enumerate(mem):
+            ens.set_state(i, Q0, [0.0, 0.0, 0.0, 0.0])
+        tt = time.perf_counter()
+        shapes = ens.tune(tune_steps)
+        tt = time.perf_counter() - tt
+        ens.step(warmup)
+        t0 = time.perf_counter()
+        ens.step(steps)
+        wall = time.perf_counter() - t0
+        states[prec] = [ens.get_state(i)[0] for i in range(len(mem))]
+        ens.close()
+        res[prec] = {"value": ndof * 5 * steps / wall, "wall_s": wall,
+                     "tile_shapes": {str(n): ("strips" if v == abi.SD2D_TILE_STRIPS else "blocks")
+                                     for n, v in shapes.items()},
+                     "tune_s": tt}
+        print(f"[bench] c4 precision {prec}: {steps} steps in {wall:.2f}s, shapes {shapes}",
+              file=sys.stderr, flush=True)
+    # the same runs one after another (mixed), same number of steps
+    seq = 0.0
+    for c, Q0, _ in mem:
+        g = Sd2d(c, precision=abi.SD2D_MIXED)
+        g.set_state(Q0, [0.0, 0.0, 0.0, 0.0])
+        g.step(warmup)
+        t0 = time.perf_counter()
+        g.step(steps)
+        seq += time.perf_counter() - t0
+        g.close()
+    print(f"[bench] c4 one by one: {seq:.2f}s", file=sys.stderr, flush=True)
+    # mixed vs FP64 after warmup + steps, per variable, worst member
+    l2 = np.zeros(9)
+    nfin = 0
+    for a, b in zip(states[abi.SD2D_MIXED], states[abi.SD2D_FP64]):
+        if not (np.all(np.isfinite(a)) and np.all(np.isfinite(b))):
+            continue
+        nfin += 1
+        a = a.reshape(9, -1, order="F"); b = b.reshape(9, -1, order="F")
+        nb = np.sqrt((b * b).sum(axis=1))
+        den = nb.copy()
+        den[[3, 7, 8]] = nb.max()
+        den[den == 0] = nb.max()
+        l2 = np.maximum(l2, np.sqrt(((a - b) ** 2).sum(axis=1)) / den)
+    vm, v64 = res[abi.SD2D_MIXED]["value"], res[abi.SD2D_FP64]["value"]
+    return {"metric": METRIC, "workload": "C4 ensemble: 64 cylinder runs, Nr x Ntheta in "
+            "{32x64..256x512}, p in {2,3,4}, Mach U[1.2,3], Alfven Mach U[2,8], field angle "
+            "U[0,90] deg (seed 4), RK_S5P3 at CFL 0.15",
+            "members": len(mem), "dof": ndof, "steps": steps, "warmup": warmup,
+            "mixed_value": vm, "fp64_value": v64, "unit": "DOF-stage/s",
+            "speedup_mixed_vs_fp64": vm / v64,
+            "one_by_one_mixed_value": ndof * 5 * steps / seq,
+            "speedup_ensemble_vs_one_by_one": (ndof * 5 * steps / res[abi.SD2D_MIXED]["wall_s"])
+            / (ndof * 5 * steps / seq),
+            "tile_shapes_mixed": res[abi.SD2D_MIXED]["tile_shapes"],
+            "tile_shapes_fp64": res[abi.SD2D_FP64]["tile_shapes"],
+            "finite_members": nfin,
+            "l2_rel_mixed_vs_fp64_worst_member": [float(x) for x in l2]}
+
+
 def traffic_from_profiles(kernel_tag: str):
     """HBM bytes per launch of the stage kernel measured with rocprofv3
     --pmc (FETCH_SIZE, WRITE_SIZE), scaled by the copy-kernel calibration,
@@ -98,7 +171,8 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3"])
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4"],
+                    help="c4: the 64-member ensemble, reported on a line of its own")
     ap.add_argument("--t-end", type=float, default=None,
                     help="instead of the throughput run: FP64 and mixed runs to this "
                          "time, reporting the mixed form's L2 difference")
@@ -118,6 +192,9 @@ def main() -> int:
     from sddc_amd import abi, cases
     from sddc_amd.solver import Sd2d
 
+    if args.workload == "c4":
+        print(json.dumps(run_c4(args.steps, args.warmup)), flush=True)
+        return 0
     t = time.perf_counter()
     (case, Q0), desc = build_workload(args.workload)
     print(f"[bench] workload {args.workload}: {case.ncell} cells, N={case.N}, "

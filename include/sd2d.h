@@ -221,6 +221,42 @@ int sd2d_read_restart(sd2d_ctx *ctx, const char *path);
 int sd2d_set_profiling(sd2d_ctx *ctx, int32_t on);
 int sd2d_get_profile(sd2d_ctx *ctx, double *stage_ms, int32_t *nlaunch);
 
+/*
+ * Ensembles: many independent runs of the path stepped together, each with
+ * its own mesh, order N, free stream, time step, state, running maxima and
+ * time.  No counterpart in the reference, which runs one case per program;
+ * each member follows the per-step contract of sd2d_step (RK_S5P3 only,
+ * inviscid path, one precision and ARTIV/vismode for all members).
+ * Members of one order share the launches of every RK stage; one whole
+ * step of all members is a HIP graph replayed per sd2d_ens_step.  The
+ * mesh and params arrays passed to sd2d_ens_create are read again by
+ * sd2d_ens_tune and must stay valid until the ensemble is destroyed.
+ *
+ *   sd2d_ens_create       nmem members: meshes[i], params[i] (tile_shape of
+ *                         the first member of each order is that order's
+ *                         starting shape)
+ *   sd2d_ens_set_state / sd2d_ens_get_state / sd2d_ens_set_time /
+ *   sd2d_ens_get_time     member i, as sd2d_set_state / sd2d_get_state /
+ *                         sd2d_set_time / sd2d_get_diag's ctime and iter
+ *   sd2d_ens_step         nsteps RK_S5P3 steps of every member
+ *   sd2d_ens_tune         autotuner: for each order, steps every tile shape
+ *                         (SD2D_TILE_STRIPS, SD2D_TILE_BLOCKS) trial_steps
+ *                         times from the current states, keeps the fastest
+ *                         and restores the states; shapes[N-3] returns the
+ *                         choice for each order present (others untouched)
+ */
+typedef struct sd2d_ens sd2d_ens;
+int sd2d_ens_create(int32_t nmem, const sd2d_mesh *meshes, const sd2d_params *params,
+                    sd2d_ens **out);
+void sd2d_ens_destroy(sd2d_ens *ens);
+const char *sd2d_ens_last_error(const sd2d_ens *ens);
+int sd2d_ens_set_state(sd2d_ens *ens, int32_t member, const double *q, const sd2d_maxima *maxima);
+int sd2d_ens_get_state(sd2d_ens *ens, int32_t member, double *q, sd2d_maxima *maxima);
+int sd2d_ens_set_time(sd2d_ens *ens, int32_t member, double ctime, int32_t iter);
+int sd2d_ens_get_time(sd2d_ens *ens, int32_t member, double *ctime, int32_t *iter);
+int sd2d_ens_step(sd2d_ens *ens, int32_t nsteps);
+int sd2d_ens_tune(sd2d_ens *ens, int32_t trial_steps, int32_t *shapes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -105,6 +105,46 @@ def orszag_tang(n: int, N: int = 4, rho_pert: float = 1e-4, seed: int = 3,
     return case, Q0
 
 
+def c4_ensemble(nmem: int = 64, seed: int = 4, r1: float = 15.0, ramp: float = 2.0,
+                cfl: float = 0.15):
+    """The C4 ensemble: nmem cylinder runs with Nr x Ntheta drawn from
+    {32x64, 64x128, 128x256, 256x512}, p from {2,3,4} (N = p+1), free-stream
+    Mach from U[1.2, 3.0], Alfven Mach from U[2, 8] and field angle from
+    U[0, 90] degrees, all from one seed.  Every member starts at rest on the
+    wall with the velocity rising over `ramp` radii and steps at CFL `cfl`
+    (the unlimited scheme of the reference stays finite over 500 steps for
+    the strongest-field, highest-Mach draws at CFL 0.15, not at 0.3).
+    Meshes and metrics are shared between members of equal size and order.
+    Returns a list of (case, Q0, draw) with draw the drawn parameters."""
+    import dataclasses
+    rng = np.random.default_rng(seed)
+    sizes = [(32, 64), (64, 128), (128, 256), (256, 512)]
+    base = {}
+    out = []
+    for i in range(nmem):
+        nr, nt = sizes[int(rng.integers(0, 4))]
+        N = int(rng.integers(2, 5)) + 1
+        mach = float(rng.uniform(1.2, 3.0))
+        ma = float(rng.uniform(2.0, 8.0))
+        ang = float(rng.uniform(0.0, 90.0))
+        key = (nr, nt, N)
+        if key not in base:
+            base[key] = M.build_case(M.ogrid(nr, nt, 0.5, r1), M.RunParams(N=N))
+        gam = M.GAM
+        rinf, pinf = 1.0, 1.0 / gam
+        u = mach * math.sqrt(gam * pinf / rinf)
+        bmag = u * math.sqrt(rinf) / ma
+        th = math.radians(ang)
+        p = M.RunParams(N=N, rinf=rinf, pinf=pinf, uinf=u, vinf=0.0,
+                        bxinf=bmag * math.cos(th), byinf=bmag * math.sin(th))
+        case = dataclasses.replace(base[key], params=p)
+        Q0 = M.freestream_state(case, wall_ramp=(0.5, ramp))
+        case.params.dt = cfl_dt(case, Q0, cfl)
+        out.append((case, Q0, {"nr": nr, "ntheta": nt, "N": N, "mach": mach,
+                               "alfven_mach": ma, "b_angle_deg": ang}))
+    return out
+
+
 def dof(case: M.Case) -> int:
     return case.ncell * case.N * case.N
 

@@ -48,6 +48,13 @@ struct sd2d_ctx {
   double *normbuf = nullptr, *normout = nullptr;
   double *stagef = nullptr;                     // Fortran-layout staging (Q or resid)
   double *nablaf = nullptr;                     // Fortran-layout nablaQs
+  // ensemble group: nmem members share the launches (per-tile member and
+  // valid-cell count, per-member dt); 1 and null otherwise.  phm holds the
+  // free stream of every member (one entry for one run).
+  int nmem = 1;
+  int *tinfo = nullptr;
+  double *dtm = nullptr;
+  Phys *phm = nullptr;
   // viscous path (vismode 1, mu or eta0 non-zero)
   bool vis = false;
   double *gq = nullptr, *dei = nullptr, *visres = nullptr, *metw = nullptr;
@@ -120,11 +127,13 @@ __global__ void k_t2f(const double *__restrict__ t, double *__restrict__ f,
   }
 }
 
-// running maxima: g = max over slots (single block)
+// running maxima: g = max over slots (one block per ensemble member)
 __global__ void k_finalize_max(const unsigned long long *__restrict__ slots,
                                double *__restrict__ gmax) {
   __shared__ unsigned long long s[2][NSLOT];
   const int t = threadIdx.x;
+  slots += 2 * NSLOT * (size_t)blockIdx.x;
+  gmax += 2 * blockIdx.x;
   s[0][t] = slots[2 * t];
   s[1][t] = slots[2 * t + 1];
   __syncthreads();
@@ -174,7 +183,7 @@ static hipError_t launch_stage_mx_t(sd2d_ctx *c, const StageArgs &a) {
   constexpr size_t lds = stage_mx_lds_bytes<N, MODE>();
   static_assert(lds <= 65536, "stage kernel LDS above 64 KiB");
   hipLaunchKernelGGL((stage_kernel_mx<N, MODE>), dim3(c->ntile), dim3(STAGE_THREADS), lds,
-                     c->stream, a, c->opsf, c->ph, c->metf);
+                     c->stream, a, c->opsf, c->metf);
   return hipGetLastError();
 }
 
@@ -187,7 +196,7 @@ static hipError_t launch_stage_t(sd2d_ctx *c, const StageArgs &a) {
   static_assert(lds <= 65536, "stage kernel LDS above 64 KiB");
   static_assert(2 * N * Geo<N>::CPW <= STAGE_THREADS, "one thread per grid line");
   hipLaunchKernelGGL((stage_kernel<N, MODE>), dim3(c->ntile), dim3(STAGE_THREADS), lds,
-                     c->stream, a, c->ops, c->ph);
+                     c->stream, a, c->ops);
   return hipGetLastError();
 }
 
@@ -303,12 +312,109 @@ static StageArgs base_args(sd2d_ctx *c) {
   a.ntile = c->ntile;
   a.sm_ind = c->sm_ind;
   a.s2c = c->s2c;
+  a.tinfo = c->tinfo;
+  a.dtm = c->dtm;
+  a.phm = c->phm;
   return a;
 }
 
 static int finalize_stage(sd2d_ctx *c) {
-  hipLaunchKernelGGL(k_finalize_max, dim3(1), dim3(NSLOT), 0, c->stream, c->slots, c->gmax);
+  hipLaunchKernelGGL(k_finalize_max, dim3(c->nmem), dim3(NSLOT), 0, c->stream, c->slots, c->gmax);
   HIPCHK(c, hipGetLastError());
+  return SD2D_OK;
+}
+
+// Device allocations and uploads of a context from its host tables.  An
+// ensemble group passes its members' per-tile info, time steps and free
+// streams (nmem > 1 entries of the maxima); one run passes null.  On error
+// the context keeps what was allocated (sd2d_destroy frees it).
+static int ctx_upload(sd2d_ctx *c, const HostTables &tb, int nwall, int nmem,
+                      const std::vector<int> *tinfo, const std::vector<double> *dtm,
+                      const std::vector<Phys> *phm) {
+  const int nt = c->ntile, N = c->N, CPW = c->cpw, N2 = N * N, nc = c->ncell;
+  const bool vis = c->vis;
+  const sd2d_params *p = &c->p;
+  const std::vector<int> &ft = tb.ft;
+  const std::vector<double> &met = tb.met;
+  const std::vector<double> &av = tb.av;
+
+#define CK(call)                                                                   \
+  do {                                                                             \
+    hipError_t e_ = (call);                                                        \
+    if (e_ != hipSuccess) {                                                        \
+      c->err = std::string(#call) + ": " + hipGetErrorString(e_);                  \
+      return e_ == hipErrorOutOfMemory ? SD2D_ENOMEM : SD2D_EHIP;                  \
+    }                                                                              \
+  } while (0)
+  CK(hipGetDevice(&c->device));
+  CK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  const size_t qn = (size_t)nt * 9 * N2 * CPW;
+  for (int i = 0; i < 3; ++i) CK(hipMalloc(&c->q[i], qn * sizeof(double)));
+  if (p->k_stage == 5)
+    for (int i = 0; i < 4; ++i) CK(hipMalloc(&c->fsr[i], qn * sizeof(double)));
+  CK(hipMalloc(&c->met, met.size() * sizeof(double)));
+  std::vector<float> metf;
+  if (p->precision == SD2D_MIXED) {
+    metf.assign(met.begin(), met.end());
+    CK(hipMalloc(&c->metf, metf.size() * sizeof(float)));
+  }
+  CK(hipMalloc(&c->ftab, ft.size() * sizeof(int)));
+  CK(hipMalloc(&c->torder, tb.torder.size() * sizeof(int)));
+  CK(hipMalloc(&c->c2s, tb.c2s.size() * sizeof(int)));
+  CK(hipMalloc(&c->s2c, tb.s2c.size() * sizeof(int)));
+  CK(hipMalloc(&c->avtab, av.size() * sizeof(double)));
+  CK(hipMalloc(&c->gmax, 2 * (size_t)nmem * sizeof(double)));
+  CK(hipMalloc(&c->slots, 2 * NSLOT * (size_t)nmem * sizeof(unsigned long long)));
+  CK(hipMalloc(&c->sm_ind, 2 * (size_t)nc * sizeof(double)));
+  CK(hipMalloc(&c->normbuf, 2 * (size_t)nt * sizeof(double)));
+  CK(hipMalloc(&c->normout, 2 * sizeof(double)));
+  CK(hipMalloc(&c->stagef, (size_t)nc * 9 * N2 * sizeof(double)));
+  if (vis) {
+    CK(hipMalloc(&c->gq, 2 * qn * sizeof(double)));
+    CK(hipMalloc(&c->visres, qn * sizeof(double)));
+    CK(hipMalloc(&c->metw, tb.metw.size() * sizeof(double)));
+    CK(hipMemcpyAsync(c->metw, tb.metw.data(), tb.metw.size() * sizeof(double),
+                      hipMemcpyHostToDevice, c->stream));
+    if (nwall > 0)   // comp_grad_internal_energy runs only with walls (compflux.f90:16)
+      CK(hipMalloc(&c->dei, (size_t)nt * 2 * N2 * CPW * sizeof(double)));
+  }
+  CK(hipMemcpyAsync(c->met, met.data(), met.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  if (c->metf)
+    CK(hipMemcpyAsync(c->metf, metf.data(), metf.size() * sizeof(float), hipMemcpyHostToDevice,
+                      c->stream));
+  CK(hipMemcpyAsync(c->ftab, ft.data(), ft.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  CK(hipMemcpyAsync(c->torder, tb.torder.data(), tb.torder.size() * sizeof(int),
+                    hipMemcpyHostToDevice, c->stream));
+  CK(hipMemcpyAsync(c->c2s, tb.c2s.data(), tb.c2s.size() * sizeof(int), hipMemcpyHostToDevice,
+                    c->stream));
+  CK(hipMemcpyAsync(c->s2c, tb.s2c.data(), tb.s2c.size() * sizeof(int), hipMemcpyHostToDevice,
+                    c->stream));
+  CK(hipMemcpyAsync(c->avtab, av.data(), av.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  for (int i = 0; i < 3; ++i) CK(hipMemsetAsync(c->q[i], 0, qn * sizeof(double), c->stream));
+  CK(hipMemsetAsync(c->sm_ind, 0, 2 * (size_t)nc * sizeof(double), c->stream));
+  if (tinfo) {
+    CK(hipMalloc(&c->tinfo, tinfo->size() * sizeof(int)));
+    CK(hipMalloc(&c->dtm, dtm->size() * sizeof(double)));
+    CK(hipMemcpyAsync(c->tinfo, tinfo->data(), tinfo->size() * sizeof(int), hipMemcpyHostToDevice,
+                      c->stream));
+    CK(hipMemcpyAsync(c->dtm, dtm->data(), dtm->size() * sizeof(double), hipMemcpyHostToDevice,
+                      c->stream));
+  }
+  {
+    const std::vector<Phys> one{tb.ph};
+    const std::vector<Phys> &ph = phm ? *phm : one;
+    CK(hipMalloc(&c->phm, ph.size() * sizeof(Phys)));
+    CK(hipMemcpyAsync(c->phm, ph.data(), ph.size() * sizeof(Phys), hipMemcpyHostToDevice,
+                      c->stream));
+  }
+  c->nmem = nmem;
+  for (int k = 0; k < nmem; ++k) {
+    hipLaunchKernelGGL(k_set_max, dim3(1), dim3(NSLOT), 0, c->stream, c->slots + 2 * NSLOT * (size_t)k,
+                       c->gmax + 2 * k, 0.0, 0.0, 0.0, 0.0);
+    CK(hipGetLastError());
+  }
+  CK(hipStreamSynchronize(c->stream));
+#undef CK
   return SD2D_OK;
 }
 
@@ -354,71 +460,10 @@ int sd2d_create(const sd2d_mesh *m, const sd2d_params *p, sd2d_ctx **out) {
     }
   c->ph = tb.ph;
   c->vis = vis;
-  const int nt = c->ntile;
-  const std::vector<int> &ft = tb.ft;
-  const std::vector<double> &met = tb.met;
-  const std::vector<double> &av = tb.av;
-
-  auto bail = [&](int code) { g_err = c->err; sd2d_destroy(c); return code; };
-#define CK(call)                                                                   \
-  do {                                                                             \
-    hipError_t e_ = (call);                                                        \
-    if (e_ != hipSuccess) {                                                        \
-      c->err = std::string(#call) + ": " + hipGetErrorString(e_);                  \
-      return bail(e_ == hipErrorOutOfMemory ? SD2D_ENOMEM : SD2D_EHIP);            \
-    }                                                                              \
-  } while (0)
-  CK(hipGetDevice(&c->device));
-  CK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-  const size_t qn = (size_t)nt * 9 * N2 * CPW;
-  for (int i = 0; i < 3; ++i) CK(hipMalloc(&c->q[i], qn * sizeof(double)));
-  if (p->k_stage == 5)
-    for (int i = 0; i < 4; ++i) CK(hipMalloc(&c->fsr[i], qn * sizeof(double)));
-  CK(hipMalloc(&c->met, met.size() * sizeof(double)));
-  std::vector<float> metf;
-  if (p->precision == SD2D_MIXED) {
-    metf.assign(met.begin(), met.end());
-    CK(hipMalloc(&c->metf, metf.size() * sizeof(float)));
+  {
+    const int r = ctx_upload(c, tb, m->nwall, 1, nullptr, nullptr, nullptr);
+    if (r) { g_err = c->err; sd2d_destroy(c); return r; }
   }
-  CK(hipMalloc(&c->ftab, ft.size() * sizeof(int)));
-  CK(hipMalloc(&c->torder, tb.torder.size() * sizeof(int)));
-  CK(hipMalloc(&c->c2s, tb.c2s.size() * sizeof(int)));
-  CK(hipMalloc(&c->s2c, tb.s2c.size() * sizeof(int)));
-  CK(hipMalloc(&c->avtab, av.size() * sizeof(double)));
-  CK(hipMalloc(&c->gmax, 2 * sizeof(double)));
-  CK(hipMalloc(&c->slots, 2 * NSLOT * sizeof(unsigned long long)));
-  CK(hipMalloc(&c->sm_ind, 2 * (size_t)nc * sizeof(double)));
-  CK(hipMalloc(&c->normbuf, 2 * (size_t)nt * sizeof(double)));
-  CK(hipMalloc(&c->normout, 2 * sizeof(double)));
-  CK(hipMalloc(&c->stagef, (size_t)nc * 9 * N2 * sizeof(double)));
-  if (vis) {
-    CK(hipMalloc(&c->gq, 2 * qn * sizeof(double)));
-    CK(hipMalloc(&c->visres, qn * sizeof(double)));
-    CK(hipMalloc(&c->metw, tb.metw.size() * sizeof(double)));
-    CK(hipMemcpyAsync(c->metw, tb.metw.data(), tb.metw.size() * sizeof(double),
-                      hipMemcpyHostToDevice, c->stream));
-    if (m->nwall > 0)   // comp_grad_internal_energy runs only with walls (compflux.f90:16)
-      CK(hipMalloc(&c->dei, (size_t)nt * 2 * N2 * CPW * sizeof(double)));
-  }
-  CK(hipMemcpyAsync(c->met, met.data(), met.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
-  if (c->metf)
-    CK(hipMemcpyAsync(c->metf, metf.data(), metf.size() * sizeof(float), hipMemcpyHostToDevice,
-                      c->stream));
-  CK(hipMemcpyAsync(c->ftab, ft.data(), ft.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
-  CK(hipMemcpyAsync(c->torder, tb.torder.data(), tb.torder.size() * sizeof(int),
-                    hipMemcpyHostToDevice, c->stream));
-  CK(hipMemcpyAsync(c->c2s, tb.c2s.data(), tb.c2s.size() * sizeof(int), hipMemcpyHostToDevice,
-                    c->stream));
-  CK(hipMemcpyAsync(c->s2c, tb.s2c.data(), tb.s2c.size() * sizeof(int), hipMemcpyHostToDevice,
-                    c->stream));
-  CK(hipMemcpyAsync(c->avtab, av.data(), av.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
-  for (int i = 0; i < 3; ++i) CK(hipMemsetAsync(c->q[i], 0, qn * sizeof(double), c->stream));
-  CK(hipMemsetAsync(c->sm_ind, 0, 2 * (size_t)nc * sizeof(double), c->stream));
-  hipLaunchKernelGGL(k_set_max, dim3(1), dim3(NSLOT), 0, c->stream, c->slots, c->gmax, 0.0, 0.0,
-                     0.0, 0.0);
-  CK(hipGetLastError());
-  CK(hipStreamSynchronize(c->stream));
-#undef CK
   *out = c;
   return SD2D_OK;
 }
@@ -430,7 +475,7 @@ void sd2d_destroy(sd2d_ctx *c) {
   for (double *b : c->fsr) if (b) (void)hipFree(b);
   void *ptrs[] = {c->met, c->metf, c->ftab, c->torder, c->avtab, c->gmax, c->slots, c->sm_ind, c->normbuf,
                   c->normout, c->stagef, c->nablaf, c->gq, c->dei, c->visres, c->metw,
-                  c->c2s, c->s2c};
+                  c->c2s, c->s2c, c->tinfo, c->dtm, c->phm};
   for (void *b : ptrs) if (b) (void)hipFree(b);
   for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -732,3 +777,5 @@ int sd2d_get_av(sd2d_ctx *c, double *sm_ind, double *muav, double *epsilon0) {
 }
 
 }  // extern "C"
+
+#include "sd2d_ens.hpp"

@@ -55,6 +55,12 @@ struct StageArgs {
   const double *avtab;      // hlow[isl][is] then hhigh[is][isl] (AV.f90:18-63)
   const double *visres;     // viscous divergence dFv + dGv (tile layout), M_VIS
   const int *s2c;           // slot -> mesh cell, for the cell-indexed outputs
+  // ensemble of runs (null for one run): per tile (member << 6 | valid
+  // cells); per member dt (qout then uses cr * dt), and gmax / slots hold
+  // one entry per member
+  const int *tinfo;
+  const double *dtm;
+  const Phys *phm;          // free stream and GLM alpha, one per member
   int ncell, ntile;
 };
 
@@ -255,7 +261,7 @@ __device__ __forceinline__ void face_flux(int info, int fam, int rfp, const Phys
 }
 
 template <int N, int MODE>
-__global__ __launch_bounds__(STAGE_THREADS, 2) void stage_kernel(StageArgs a, Ops ops, Phys ph) {
+__global__ __launch_bounds__(STAGE_THREADS, 2) void stage_kernel(StageArgs a, Ops ops) {
   using G = Geo<N>;
   using LD = StageLds<N, MODE>;
   constexpr int T = G::CPW, N2 = G::N2, NF = N + 1, NT = 2 * N * T;
@@ -272,8 +278,13 @@ __global__ __launch_bounds__(STAGE_THREADS, 2) void stage_kernel(StageArgs a, Op
   const int fam = fl >= N ? 1 : 0;     // 0: xi-line (row js), 1: eta-line (column is)
   const int line = fl - fam * N;
   const int cell = tile * T + ci;
-  const bool valid = tid < NT && cell < a.ncell;
-  const double ge = a.gmax[0], gu = a.gmax[1];
+  // ensemble: the tile's member and valid-cell count; one run: member 0
+  int mem = 0, nvt = a.ncell - tile * T;
+  if (a.tinfo) { const int ti = a.tinfo[tile]; mem = ti >> 6; nvt = ti & 63; }
+  const bool valid = tid < NT && ci < nvt;
+  const double ge = a.gmax[2 * mem], gu = a.gmax[2 * mem + 1];
+  const double cr = a.dtm ? a.cr * a.dtm[mem] : a.cr;   // b * dt of the member
+  const Phys &ph = a.phm[mem];                            // the member's free stream
   const double c_h = sqrt(ge * (ge - gu));   // lagged GLM speed, compflux.f90:6
   double eig = 0.0, um = 0.0, nres = 0.0, ndiv = 0.0;
   const double *mt = a.met + (size_t)tile * G::NM * T + ci;
@@ -563,7 +574,7 @@ __global__ __launch_bounds__(STAGE_THREADS, 2) void stage_kernel(StageArgs a, Op
           double acc = a.cin * q[v];
           if (NAUX == 1) acc = fma(a.ca[0], xc[v], acc);
           if (NAUX > 1) acc += xs[v];
-          acc = fma(a.cr, R[v], acc);
+          acc = fma(cr, R[v], acc);
           a.qout[o] = acc;
           if (RSTORE) a.rstore[o] = R[v];
         }
@@ -599,7 +610,7 @@ __global__ __launch_bounds__(STAGE_THREADS, 2) void stage_kernel(StageArgs a, Op
         s1 += lds[LD::RED + 2 * NW + k];
         s2 += lds[LD::RED + 3 * NW + k];
       }
-      unsigned long long *sl = a.slots + 2 * (blockIdx.x % NSLOT);
+      unsigned long long *sl = a.slots + 2 * ((size_t)mem * NSLOT + blockIdx.x % NSLOT);
       atomicMax(sl, (unsigned long long)__double_as_longlong(e));
       atomicMax(sl + 1, (unsigned long long)__double_as_longlong(u));
       if (NORMS) {
@@ -628,7 +639,7 @@ __global__ __launch_bounds__(STAGE_THREADS, 2) void stage_kernel(StageArgs a, Op
       }
     }
     __syncthreads();
-    const int nvc = a.ncell - tile * T;      // valid cells of this tile
+    const int nvc = nvt;                      // valid cells of this tile
     for (int it = tid; it < NL2 * T; it += STAGE_THREADS) {
       const int c = it % T, t = it / T;
       const int il = t % NL, jl = t / NL;

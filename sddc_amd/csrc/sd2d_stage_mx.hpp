@@ -56,9 +56,7 @@ constexpr size_t stage_mx_lds_bytes() { return sizeof(double) * StageLdsMx<N, MO
 template <int N, int MODE>
 __global__ __launch_bounds__(STAGE_THREADS,
                              (MODE & (M_AV | M_NORMS | M_AUX4 | M_RSTORE)) ? 2 : 3)
-void stage_kernel_mx(StageArgs a, OpsF ops,
-                                                                       Phys ph,
-                                                                       const float *__restrict__ metf) {
+void stage_kernel_mx(StageArgs a, OpsF ops, const float *__restrict__ metf) {
   using G = Geo<N>;
   using LD = StageLdsMx<N, MODE>;
   constexpr int T = G::CPW, N2 = G::N2, NF = N + 1, NT = 2 * N * T;
@@ -77,8 +75,13 @@ void stage_kernel_mx(StageArgs a, OpsF ops,
   const int fam = fl >= N ? 1 : 0;
   const int line = fl - fam * N;
   const int cell = tile * T + ci;
-  const bool valid = tid < NT && cell < a.ncell;
-  const double ge = a.gmax[0], gu = a.gmax[1];
+  // ensemble: the tile's member and valid-cell count; one run: member 0
+  int mem = 0, nvt = a.ncell - tile * T;
+  if (a.tinfo) { const int ti = a.tinfo[tile]; mem = ti >> 6; nvt = ti & 63; }
+  const bool valid = tid < NT && ci < nvt;
+  const double ge = a.gmax[2 * mem], gu = a.gmax[2 * mem + 1];
+  const double cr = a.dtm ? a.cr * a.dtm[mem] : a.cr;   // b * dt of the member
+  const Phys &ph = a.phm[mem];                            // the member's free stream
   const float c_h = float(sqrt(ge * (ge - gu)));   // compflux.f90:6
   float eig = 0.f, um = 0.f;
   double nres = 0.0, ndiv = 0.0;
@@ -277,7 +280,7 @@ void stage_kernel_mx(StageArgs a, OpsF ops,
         double acc = a.cin * q[v];
         if (NAUX == 1) acc = fma(a.ca[0], xc[v], acc);
         if (NAUX > 1) acc += xs[v];
-        acc = fma(a.cr, R[v], acc);
+        acc = fma(cr, R[v], acc);
         a.qout[o] = acc;
         if (RSTORE) a.rstore[o] = R[v];
       }
@@ -314,7 +317,7 @@ void stage_kernel_mx(StageArgs a, OpsF ops,
           s2 += lds[LD::RED + 3 * NW + k];
         }
       }
-      unsigned long long *sl = a.slots + 2 * (blockIdx.x % NSLOT);
+      unsigned long long *sl = a.slots + 2 * ((size_t)mem * NSLOT + blockIdx.x % NSLOT);
       atomicMax(sl, (unsigned long long)__double_as_longlong(em));
       atomicMax(sl + 1, (unsigned long long)__double_as_longlong(ummax));
       if (NORMS) {
@@ -339,7 +342,7 @@ void stage_kernel_mx(StageArgs a, OpsF ops,
       }
     }
     __syncthreads();
-    const int nvc = a.ncell - tile * T;
+    const int nvc = nvt;                      // valid cells of this tile
     for (int it = tid; it < NL2 * T; it += STAGE_THREADS) {
       const int c = it % T, t = it / T;
       const int il = t % NL, jl = t / NL;

@@ -27,7 +27,10 @@ EXPORTS = ("sd2d_create", "sd2d_destroy", "sd2d_last_error", "sd2d_set_state",
            "sd2d_get_state", "sd2d_set_time", "sd2d_compresid",
            "sd2d_compresid_q", "sd2d_step", "sd2d_get_diag", "sd2d_get_av",
            "sd2d_write_restart", "sd2d_read_restart",
-           "sd2d_set_profiling", "sd2d_get_profile")
+           "sd2d_set_profiling", "sd2d_get_profile",
+           "sd2d_ens_create", "sd2d_ens_destroy", "sd2d_ens_last_error",
+           "sd2d_ens_set_state", "sd2d_ens_get_state", "sd2d_ens_set_time",
+           "sd2d_ens_get_time", "sd2d_ens_step", "sd2d_ens_tune")
 
 
 class Sd2dError(RuntimeError):
@@ -61,8 +64,22 @@ def load_library(path: str = LIBPATH):
     L.sd2d_read_restart.argtypes = [vp, C.c_char_p]
     L.sd2d_set_profiling.argtypes = [vp, C.c_int32]
     L.sd2d_get_profile.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int32)]
+    L.sd2d_ens_create.argtypes = [C.c_int32, C.POINTER(abi.Sd2dMesh),
+                                  C.POINTER(abi.Sd2dParams), C.POINTER(vp)]
+    L.sd2d_ens_destroy.argtypes = [vp]
+    L.sd2d_ens_destroy.restype = None
+    L.sd2d_ens_last_error.argtypes = [vp]
+    L.sd2d_ens_last_error.restype = C.c_char_p
+    L.sd2d_ens_set_state.argtypes = [vp, C.c_int32, vp, C.POINTER(abi.Sd2dMaxima)]
+    L.sd2d_ens_get_state.argtypes = [vp, C.c_int32, vp, C.POINTER(abi.Sd2dMaxima)]
+    L.sd2d_ens_set_time.argtypes = [vp, C.c_int32, C.c_double, C.c_int32]
+    L.sd2d_ens_get_time.argtypes = [vp, C.c_int32, C.POINTER(C.c_double),
+                                    C.POINTER(C.c_int32)]
+    L.sd2d_ens_step.argtypes = [vp, C.c_int32]
+    L.sd2d_ens_tune.argtypes = [vp, C.c_int32, vp]
     for f in EXPORTS:
-        if f not in ("sd2d_destroy", "sd2d_last_error"):
+        if f not in ("sd2d_destroy", "sd2d_last_error", "sd2d_ens_destroy",
+                     "sd2d_ens_last_error"):
             getattr(L, f).restype = C.c_int
     _lib = L
     return L
@@ -160,3 +177,75 @@ class Sd2d:
         e = C.c_double()
         self._check(self.L.sd2d_get_av(self.h, _ptr(s), _ptr(m), C.byref(e)), "sd2d_get_av")
         return s, m, e.value
+
+
+class Sd2dEnsemble:
+    """Many independent runs (meshes, orders, free streams, time steps)
+    stepped together on one GPU through the sd2d_ens_* entry points."""
+
+    def __init__(self, cases, precision: int = abi.SD2D_FP64,
+                 tile_shape: int = abi.SD2D_TILE_DEFAULT):
+        self.L = load_library()
+        self.cases = list(cases)
+        n = len(self.cases)
+        meshes = (abi.Sd2dMesh * n)()
+        params = (abi.Sd2dParams * n)()
+        self._keep = []   # the arrays stay alive: sd2d_ens_tune reads them again
+        for i, c in enumerate(self.cases):
+            ms, keep = abi.mesh_struct(c)
+            meshes[i] = ms
+            params[i] = abi.params_struct(c.params, c.ncell, precision, tile_shape)
+            self._keep.append(keep)
+        self._structs = (meshes, params)
+        h = C.c_void_p()
+        rc = self.L.sd2d_ens_create(n, meshes, params, C.byref(h))
+        if rc != 0:
+            raise Sd2dError(f"sd2d_ens_create: {rc}: {self.L.sd2d_ens_last_error(None).decode()}")
+        self.h = h
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise Sd2dError(f"{what}: {rc}: {self.L.sd2d_ens_last_error(self.h).decode()}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.sd2d_ens_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_state(self, i, Q, maxima=None):
+        c = self.cases[i]
+        Q = np.asfortranarray(Q, dtype=np.float64)
+        assert Q.shape == (9, c.N, c.N, c.ncell)
+        mx = abi.Sd2dMaxima(*[float(x) for x in maxima]) if maxima is not None else None
+        self._check(self.L.sd2d_ens_set_state(self.h, i, _ptr(Q), C.byref(mx) if mx else None),
+                    "sd2d_ens_set_state")
+
+    def get_state(self, i):
+        c = self.cases[i]
+        Q = np.zeros((9, c.N, c.N, c.ncell), order="F")
+        mx = abi.Sd2dMaxima()
+        self._check(self.L.sd2d_ens_get_state(self.h, i, _ptr(Q), C.byref(mx)),
+                    "sd2d_ens_get_state")
+        return Q, np.array([mx.eigvmax, mx.umax, mx.eigvmax_g, mx.umax_g])
+
+    def get_time(self, i):
+        t = C.c_double(); it = C.c_int32()
+        self._check(self.L.sd2d_ens_get_time(self.h, i, C.byref(t), C.byref(it)),
+                    "sd2d_ens_get_time")
+        return t.value, it.value
+
+    def step(self, nsteps):
+        self._check(self.L.sd2d_ens_step(self.h, int(nsteps)), "sd2d_ens_step")
+
+    def tune(self, trial_steps=3):
+        """Autotune the tile shape of every order present; returns
+        {N: shape}."""
+        sh = np.full(3, -1, dtype=np.int32)
+        self._check(self.L.sd2d_ens_tune(self.h, int(trial_steps), _ptr(sh)), "sd2d_ens_tune")
+        return {n + 3: int(v) for n, v in enumerate(sh) if v >= 0}

@@ -205,3 +205,35 @@ def test_fortran_binding_layout(tmp_path):
                                           text=True).stdout.split()]
     assert got == [C.sizeof(abi.Sd2dMesh), C.sizeof(abi.Sd2dParams),
                    C.sizeof(abi.Sd2dMaxima), C.sizeof(abi.Sd2dDiag)]
+
+
+def _ens_create(cases, mutate=None):
+    L = _lib()
+    n = len(cases)
+    meshes = (abi.Sd2dMesh * n)(); params = (abi.Sd2dParams * n)(); keep = []
+    for i, c in enumerate(cases):
+        ms, k = abi.mesh_struct(c)
+        meshes[i] = ms; keep.append(k)
+        params[i] = abi.params_struct(c.params, c.ncell)
+    if mutate:
+        mutate(meshes, params)
+    h = C.c_void_p(1234)
+    rc = L.sd2d_ens_create(n, meshes, params, C.byref(h))
+    return rc, h, L.sd2d_ens_last_error(None).decode()
+
+
+def test_ensemble_rejects_bad_input():
+    """Ensemble creation refuses, before touching the device: no members,
+    FiveStageRk, the viscous path, mixed precision settings."""
+    L = _lib()
+    h = C.c_void_p(1234)
+    assert L.sd2d_ens_create(0, None, None, C.byref(h)) == abi.SD2D_EINVAL
+    cs = [_small_case(), _small_case(N=3)]
+    rc, h, msg = _ens_create(cs, lambda m, p: setattr(p[1], "k_stage", 5))
+    assert rc == abi.SD2D_EUNSUPPORTED and "RK_S5P3" in msg and h.value is None
+    rc, _, msg = _ens_create(cs, lambda m, p: (setattr(p[0], "mu", 0.01)))
+    assert rc == abi.SD2D_EUNSUPPORTED and "inviscid" in msg
+    rc, _, msg = _ens_create(cs, lambda m, p: setattr(p[1], "precision", abi.SD2D_MIXED))
+    assert rc == abi.SD2D_EINVAL and "precision" in msg
+    rc, _, msg = _ens_create(cs, lambda m, p: setattr(m[0], "N", 7))
+    assert rc == abi.SD2D_EUNSUPPORTED and "N must be" in msg
