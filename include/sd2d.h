@@ -257,6 +257,39 @@ int sd2d_ens_get_time(sd2d_ens *ens, int32_t member, double *ctime, int32_t *ite
 int sd2d_ens_step(sd2d_ens *ens, int32_t nsteps);
 int sd2d_ens_tune(sd2d_ens *ens, int32_t trial_steps, int32_t *shapes);
 
+/*
+ * Adjoint (vector-Jacobian products), no counterpart in the reference: the
+ * FP64 inviscid path with RK_S5P3, one run (not ensembles).
+ *
+ * sd2d_fwd      nsteps RK_S5P3 steps like sd2d_step (norms off), keeping a
+ *               checkpoint (state and running maxima) every ckpt steps and
+ *               returning the objective J = (1/nsteps) sum over the steps'
+ *               outputs of the wall drag D(Q) = sum over WALL face points
+ *               of w_i p (S_out)_x: gas pressure of the cell's trace, the
+ *               outward metric vector of the face point, and w_i the
+ *               interpolatory quadrature weights of the solution-point
+ *               positions Xs along the face.
+ * sd2d_vjp      after sd2d_fwd: for the cotangents qbar_final (of the final
+ *               state, Fortran layout, or NULL) and jbar (of J) returns
+ *               qbar_init (of the initial state, Fortran layout, or NULL)
+ *               and pbar[6] (of rinf, pinf, uinf, vinf, bxinf, byinf, which
+ *               the boundary ghosts read, or NULL).  The lagged GLM speed
+ *               c_h of every evaluation is differentiated through the
+ *               running maxima it comes from, back to the flux point that
+ *               attained each maximum; chbar (5*nsteps entries, or NULL)
+ *               returns the cotangent of each evaluation's c_h.  The
+ *               context returns to the final state of sd2d_fwd.
+ * sd2d_compresid_vjp  for the residual of the current state with the
+ *               current c_h (sd2d_compresid's contract): qbar = (dR/dQ)^T
+ *               rbar, *chbar = (dR/dc_h)^T rbar and pbar[6] (Fortran
+ *               layouts; outputs may be NULL).
+ */
+int sd2d_fwd(sd2d_ctx *ctx, int32_t nsteps, int32_t ckpt, double *objective);
+int sd2d_vjp(sd2d_ctx *ctx, const double *qbar_final, double jbar, double *qbar_init,
+             double *pbar, double *chbar);
+int sd2d_compresid_vjp(sd2d_ctx *ctx, const double *rbar, double *qbar, double *chbar,
+                       double *pbar);
+
 #ifdef __cplusplus
 }
 #endif

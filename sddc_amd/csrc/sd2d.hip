@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -21,9 +22,13 @@
 #include "sd2d_stage.hpp"
 #include "sd2d_stage_mx.hpp"
 #include "sd2d_visc.hpp"
+#include "sd2d_adj.hpp"
 #include "sd2d_rkseq.hpp"
 
 using namespace sd2d;
+
+struct AdjState;                      // the adjoint's buffers (sd2d_adjhost.hpp)
+static void adj_free(AdjState *A);
 
 struct sd2d_ctx {
   int N = 0, ncell = 0, ntile = 0, cpw = 0;
@@ -55,6 +60,8 @@ struct sd2d_ctx {
   int *tinfo = nullptr;
   double *dtm = nullptr;
   Phys *phm = nullptr;
+  AdjState *adj = nullptr;              // sd2d_fwd / sd2d_vjp state
+  std::vector<double> xs;               // solution-point positions (face quadrature)
   // viscous path (vismode 1, mu or eta0 non-zero)
   bool vis = false;
   double *gq = nullptr, *dei = nullptr, *visres = nullptr, *metw = nullptr;
@@ -460,6 +467,7 @@ int sd2d_create(const sd2d_mesh *m, const sd2d_params *p, sd2d_ctx **out) {
     }
   c->ph = tb.ph;
   c->vis = vis;
+  c->xs.assign(m->xs, m->xs + N);
   {
     const int r = ctx_upload(c, tb, m->nwall, 1, nullptr, nullptr, nullptr);
     if (r) { g_err = c->err; sd2d_destroy(c); return r; }
@@ -471,6 +479,7 @@ int sd2d_create(const sd2d_mesh *m, const sd2d_params *p, sd2d_ctx **out) {
 void sd2d_destroy(sd2d_ctx *c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  adj_free(c->adj);
   for (double *b : c->q) if (b) (void)hipFree(b);
   for (double *b : c->fsr) if (b) (void)hipFree(b);
   void *ptrs[] = {c->met, c->metf, c->ftab, c->torder, c->avtab, c->gmax, c->slots, c->sm_ind, c->normbuf,
@@ -779,3 +788,4 @@ int sd2d_get_av(sd2d_ctx *c, double *sm_ind, double *muav, double *epsilon0) {
 }  // extern "C"
 
 #include "sd2d_ens.hpp"
+#include "sd2d_adjhost.hpp"

@@ -191,20 +191,20 @@ __device__ __forceinline__ void rusanov(const T *QL, const T *QR, T nx, T ny,
 // and the one of type `bc` is selected element by element, so the ghost
 // state stays in registers (boundary points are a vanishing fraction of
 // all face points, so the extra arithmetic costs nothing measurable).
+// The free stream enters as values of T (rinf, pinf, uinf, vinf, bxinf,
+// byinf) so that the adjoint can differentiate the ghost with respect to it.
 template <typename T>
-__device__ __forceinline__ void bc_ghost(int bc, const T *Ql, T nx, T ny,
-                                         const Phys &ph, T *Qr) {
+__device__ __forceinline__ void bc_ghost_p(int bc, const T *Ql, T nx, T ny, T rinf, T pinf,
+                                           T ui, T vi, T bxi, T byi, int vismode, T *Qr) {
   const T g1 = T(GAM - 1.0);
   const T ri = rcp(Ql[0]);
   const T uw = Ql[1] * ri, vw = Ql[2] * ri, ww = Ql[3] * ri;
   const T im = rcp(sqrt(nx * nx + ny * ny));
   const T vn = (uw * nx + vw * ny) * im;
   const T ur = uw - T(2) * vn * nx * im, vr = vw - T(2) * vn * ny * im;  // reflected
-  const T rinf = T(ph.rinf), bxi = T(ph.bxinf), byi = T(ph.byinf);
-  const T p_e = T(ph.pinf) / g1;
+  const T p_e = pinf / g1;
   const T half_psi2 = T(0.5) * Ql[8] * Ql[8];
   // INLE (bcflux.f90:51-72)
-  const T ui = T(ph.uinf), vi = T(ph.vinf);
   const T e_in = p_e + T(0.5) * rinf * (ui * ui + vi * vi) +
                  T(0.5) * (bxi * bxi + byi * byi) + half_psi2;
   // OUTL (bcflux.f90:122-143)
@@ -216,7 +216,7 @@ __device__ __forceinline__ void bc_ghost(int bc, const T *Ql, T nx, T ny,
   const T e_sy = p_e + T(0.5) * rcp(rs) * (ms1 * ms1 + ms2 * ms2 + ms3 * ms3) +
                  T(0.5) * (bxi * bxi + byi * byi + Ql[7] * Ql[7] + Ql[8] * Ql[8]);
   // WALL (bcflux.f90:196-227): slip reflection for vismode 0, mirror else
-  const bool slip = ph.vismode == 0;
+  const bool slip = vismode == 0;
   const T mw1 = slip ? Ql[0] * ur : -Ql[1];
   const T mw2 = slip ? Ql[0] * vr : -Ql[2];
   const T mw3 = -Ql[3];
@@ -231,6 +231,13 @@ __device__ __forceinline__ void bc_ghost(int bc, const T *Ql, T nx, T ny,
   Qr[6] = (in || out || sy) ? byi : (wall ? -Ql[6] : Ql[6]);
   Qr[7] = (in || out) ? T(0) : ((wall || sy) ? -Ql[7] : Ql[7]);
   Qr[8] = out ? T(0) : Ql[8];
+}
+
+template <typename T>
+__device__ __forceinline__ void bc_ghost(int bc, const T *Ql, T nx, T ny,
+                                         const Phys &ph, T *Qr) {
+  bc_ghost_p<T>(bc, Ql, nx, ny, T(ph.rinf), T(ph.pinf), T(ph.uinf), T(ph.vinf), T(ph.bxinf),
+                T(ph.byinf), ph.vismode, Qr);
 }
 
 }  // namespace sd2d

@@ -30,7 +30,8 @@ EXPORTS = ("sd2d_create", "sd2d_destroy", "sd2d_last_error", "sd2d_set_state",
            "sd2d_set_profiling", "sd2d_get_profile",
            "sd2d_ens_create", "sd2d_ens_destroy", "sd2d_ens_last_error",
            "sd2d_ens_set_state", "sd2d_ens_get_state", "sd2d_ens_set_time",
-           "sd2d_ens_get_time", "sd2d_ens_step", "sd2d_ens_tune")
+           "sd2d_ens_get_time", "sd2d_ens_step", "sd2d_ens_tune",
+           "sd2d_fwd", "sd2d_vjp", "sd2d_compresid_vjp")
 
 
 class Sd2dError(RuntimeError):
@@ -77,6 +78,9 @@ def load_library(path: str = LIBPATH):
                                     C.POINTER(C.c_int32)]
     L.sd2d_ens_step.argtypes = [vp, C.c_int32]
     L.sd2d_ens_tune.argtypes = [vp, C.c_int32, vp]
+    L.sd2d_fwd.argtypes = [vp, C.c_int32, C.c_int32, C.POINTER(C.c_double)]
+    L.sd2d_vjp.argtypes = [vp, vp, C.c_double, vp, vp, vp]
+    L.sd2d_compresid_vjp.argtypes = [vp, vp, vp, C.POINTER(C.c_double), vp]
     for f in EXPORTS:
         if f not in ("sd2d_destroy", "sd2d_last_error", "sd2d_ens_destroy",
                      "sd2d_ens_last_error"):
@@ -171,6 +175,35 @@ class Sd2d:
         ms = C.c_double(); n = C.c_int32()
         self._check(self.L.sd2d_get_profile(self.h, C.byref(ms), C.byref(n)), "sd2d_get_profile")
         return ms.value, n.value
+
+    # -- adjoint -------------------------------------------------------------
+    def fwd(self, nsteps, ckpt=20):
+        """nsteps RK_S5P3 steps with checkpoints; returns the time-averaged
+        wall drag J."""
+        J = C.c_double()
+        self._check(self.L.sd2d_fwd(self.h, int(nsteps), int(ckpt), C.byref(J)), "sd2d_fwd")
+        return J.value
+
+    def vjp(self, qbar_final=None, jbar=1.0, want_chbar=False):
+        """Cotangents of the initial state and of the free stream
+        (rinf, pinf, uinf, vinf, bxinf, byinf) after fwd(); optionally the
+        per-evaluation c_h cotangents."""
+        qf = None if qbar_final is None else np.asfortranarray(qbar_final, dtype=np.float64)
+        q0 = np.zeros((9, self.N, self.N, self.nc), order="F")
+        pb = np.zeros(6)
+        ch = np.zeros(5 * 100000) if want_chbar else None
+        self._check(self.L.sd2d_vjp(self.h, _ptr(qf), float(jbar), _ptr(q0), _ptr(pb), _ptr(ch)),
+                    "sd2d_vjp")
+        return q0, pb, ch
+
+    def compresid_vjp(self, rbar):
+        rb = np.asfortranarray(rbar, dtype=np.float64)
+        q = np.zeros((9, self.N, self.N, self.nc), order="F")
+        ch = C.c_double()
+        pb = np.zeros(6)
+        self._check(self.L.sd2d_compresid_vjp(self.h, _ptr(rb), _ptr(q), C.byref(ch), _ptr(pb)),
+                    "sd2d_compresid_vjp")
+        return q, ch.value, pb
 
     def get_av(self):
         s = np.zeros((2, self.nc), order="F"); m = np.zeros((2, self.nc), order="F")

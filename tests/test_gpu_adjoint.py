@@ -1,0 +1,93 @@
+"""Adjoint of the residual and of RK_S5P3 steps (sd2d_compresid_vjp,
+sd2d_fwd / sd2d_vjp) against central finite differences of the device's
+own forward path, through dot products with seeded directions.
+
+Residual: <Rbar, (R(Q+hV) - R(Q-hV))/2h> = <Qbar, V> and likewise for c_h
+and the six free-stream values the boundary ghosts read, to 1e-6 relative
+(h = 1e-5 relative to each variable's scale: the FD truncation error is
+~1e-10, its rounding ~1e-11).
+"""
+import numpy as np
+import pytest
+
+from tests.util import ORDERED, make_case
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ["rinf", "pinf", "uinf", "vinf", "bxinf", "byinf"]
+
+
+def _gpu(case):
+    from sddc_amd.solver import Sd2d
+    return Sd2d(case)
+
+
+@pytest.mark.parametrize("name", ORDERED)
+def test_residual_vjp_against_fd(name):
+    print(f"[gpu-adj] residual {name}", flush=True)
+    import copy
+    case, Q0 = make_case(name)
+    eg, ug = 2.3, 0.7
+    mx = [eg, ug, eg, ug]
+    rng = np.random.default_rng(5)
+    rbar = np.asfortranarray(rng.standard_normal(Q0.shape))
+    scale = np.abs(Q0).max(axis=(1, 2, 3))[:, None, None, None]
+    V = np.asfortranarray(rng.standard_normal(Q0.shape)) * scale
+    g = _gpu(case)
+
+    def R(Q, m=mx, gg=None):
+        gg = gg or g
+        gg.set_state(Q, m)
+        return gg.compresid(want_nabla=False)[0]
+
+    g.set_state(Q0, mx)
+    qb, chb, pb = g.compresid_vjp(rbar)
+    h = 1e-5
+    fd = np.sum(rbar * (R(Q0 + h * V) - R(Q0 - h * V))) / (2 * h)
+    ad = np.sum(qb * V)
+    assert abs(ad - fd) <= 1e-6 * abs(fd), (ad, fd)
+    # c_h = sqrt(eg (eg - ug)) through eigvmax_g
+    ch = np.sqrt(eg * (eg - ug))
+    he = 1e-6
+    fdc = np.sum(rbar * (R(Q0, [eg, ug, eg + he, ug]) - R(Q0, [eg, ug, eg - he, ug]))) / (2 * he)
+    adc = chb * (2 * eg - ug) / (2 * ch)
+    assert abs(adc - fdc) <= 1e-6 * abs(fdc), (adc, fdc)
+    for j, f in enumerate(FIELDS):
+        cp, cm = copy.deepcopy(case), copy.deepcopy(case)
+        hp = 1e-6
+        setattr(cp.params, f, getattr(cp.params, f) + hp)
+        setattr(cm.params, f, getattr(cm.params, f) - hp)
+        fdp = np.sum(rbar * (R(Q0, gg=_gpu(cp)) - R(Q0, gg=_gpu(cm)))) / (2 * hp)
+        assert abs(pb[j] - fdp) <= 1e-6 * max(abs(fdp), 1e-8 * np.abs(rbar).sum()), (f, pb[j], fdp)
+    g.close()
+
+
+@pytest.mark.parametrize("name", ["ogrid4x8_N4", "channel5x4_N3", "periodic4x4_N5"])
+def test_step_vjp_against_fd(name):
+    """Three RK_S5P3 steps with checkpoints every two steps: the VJP of a
+    seeded N(0,1) cotangent on the final state plus the wall-drag
+    objective, against central differences of the device forward."""
+    print(f"[gpu-adj] steps {name}", flush=True)
+    case, Q0 = make_case(name)
+    rng = np.random.default_rng(7)
+    C = np.asfortranarray(rng.standard_normal(Q0.shape))
+    scale = np.abs(Q0).max(axis=(1, 2, 3))[:, None, None, None]
+    V = np.asfortranarray(rng.standard_normal(Q0.shape)) * scale
+    nsteps, jbar = 3, 2.0
+    g = _gpu(case)
+
+    def F(Q):
+        g.set_state(Q, [0, 0, 0, 0])
+        J = g.fwd(nsteps, 2)
+        return np.sum(C * g.get_state()[0]) + jbar * J
+
+    g.set_state(Q0, [0, 0, 0, 0])
+    g.fwd(nsteps, 2)
+    qb0, pb, ch = g.vjp(C, jbar, want_chbar=True)
+    h = 1e-6
+    fd = (F(Q0 + h * V) - F(Q0 - h * V)) / (2 * h)
+    ad = np.sum(qb0 * V)
+    print(f"[gpu-adj] {name}: adjoint {ad:.12e} fd {fd:.12e} rel {abs(ad - fd) / abs(fd):.2e}",
+          flush=True)
+    assert abs(ad - fd) <= 1e-5 * abs(fd), (ad, fd)
+    g.close()
