@@ -91,3 +91,46 @@ def test_step_vjp_against_fd(name):
           flush=True)
     assert abs(ad - fd) <= 1e-5 * abs(fd), (ad, fd)
     g.close()
+
+
+def test_parameter_gradients_against_fd():
+    """dJ/dMach and dJ/dtheta of the time-averaged wall drag (boundary
+    ghosts and initial state) against central differences with h = 1e-6;
+    the AV coefficient's gradient is exactly 0 (the sensor does not act)."""
+    print("[gpu-adj] parameter gradients", flush=True)
+    from sddc_amd.adjoint import CylinderObjective, CylinderSpec
+    o = CylinderObjective(CylinderSpec(16, 32, 4))
+    mach, ang, nsteps, ckpt = 1.5, 10.0, 20, 5
+    J, gr = o.gradients(mach, ang, nsteps, ckpt)
+    h = 1e-6                          # Mach, and the angle in radians
+    hd = np.degrees(h)
+    fdm = (o.value(mach + h, ang, nsteps, ckpt) - o.value(mach - h, ang, nsteps, ckpt)) / (2 * h)
+    fda = (o.value(mach, ang + hd, nsteps, ckpt) - o.value(mach, ang - hd, nsteps, ckpt)) / (2 * h)
+    print(f"[gpu-adj] J={J:.10e} dJ/dM adj {gr['mach']:.10e} fd {fdm:.10e}; "
+          f"dJ/dtheta adj {gr['angle_rad']:.10e} fd {fda:.10e}", flush=True)
+    assert abs(gr["mach"] - fdm) <= 1e-5 * abs(fdm)
+    assert abs(gr["angle_rad"] - fda) <= 1e-5 * abs(fda)
+    assert gr["av_coefficient"] == 0.0
+
+
+def test_torch_autograd_wrapper():
+    """The verification harness's autograd Function over sd2d_fwd/sd2d_vjp:
+    q0.grad from J.backward() against a central difference of J."""
+    print("[gpu-adj] torch wrapper (importing torch)", flush=True)
+    import torch
+    from tests.adjoint_torch import DeviceDragObjective
+    case, Q0 = make_case("ogrid4x8_N4")
+    g = _gpu(case)
+    q0 = torch.tensor(np.ascontiguousarray(Q0), dtype=torch.float64, requires_grad=True)
+    J = DeviceDragObjective.apply(q0, g, 6, 3)
+    J.backward()
+    rng = np.random.default_rng(11)
+    V = rng.standard_normal(Q0.shape) * np.abs(Q0).max(axis=(1, 2, 3))[:, None, None, None]
+    h = 1e-6
+    Jp = DeviceDragObjective.apply(torch.tensor(np.ascontiguousarray(Q0 + h * V)), g, 6, 3).item()
+    Jm = DeviceDragObjective.apply(torch.tensor(np.ascontiguousarray(Q0 - h * V)), g, 6, 3).item()
+    fd = (Jp - Jm) / (2 * h)
+    ad = float(np.sum(q0.grad.numpy() * V))
+    print(f"[gpu-adj] torch: adjoint {ad:.10e} fd {fd:.10e}", flush=True)
+    assert abs(ad - fd) <= 1e-5 * abs(fd)
+    g.close()
