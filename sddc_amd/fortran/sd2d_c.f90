@@ -59,7 +59,10 @@ module sd2d_c
   public :: sd2d_create, sd2d_destroy, sd2d_last_error, sd2d_set_state, &
        sd2d_get_state, sd2d_set_time, sd2d_compresid, sd2d_compresid_q, &
        sd2d_step, sd2d_get_diag, sd2d_get_av, sd2d_write_restart, &
-       sd2d_read_restart, sd2d_set_profiling, sd2d_get_profile, sd2d_error_message
+       sd2d_read_restart, sd2d_set_profiling, sd2d_get_profile, sd2d_error_message, &
+       sd2d_ens_create, sd2d_ens_destroy, sd2d_ens_last_error, sd2d_ens_set_state, &
+       sd2d_ens_get_state, sd2d_ens_set_time, sd2d_ens_get_time, sd2d_ens_step, &
+       sd2d_ens_tune, sd2d_fwd, sd2d_vjp, sd2d_compresid_vjp
 
   interface
      integer(c_int) function sd2d_create(mesh, params, ctx) bind(C, name="sd2d_create")
@@ -163,6 +166,104 @@ module sd2d_c
        real(c_double), intent(out), optional :: stage_ms
        integer(c_int32_t), intent(out), optional :: nlaunch
      end function sd2d_get_profile
+
+     ! ensembles: meshes(nmem), params(nmem) arrays of the derived types
+     integer(c_int) function sd2d_ens_create(nmem, meshes, params, ens) &
+          bind(C, name="sd2d_ens_create")
+       import :: c_int, c_int32_t, c_ptr, sd2d_mesh_t, sd2d_params_t
+       integer(c_int32_t), value :: nmem
+       type(sd2d_mesh_t), intent(in) :: meshes(*)
+       type(sd2d_params_t), intent(in) :: params(*)
+       type(c_ptr), intent(out) :: ens
+     end function sd2d_ens_create
+
+     subroutine sd2d_ens_destroy(ens) bind(C, name="sd2d_ens_destroy")
+       import :: c_ptr
+       type(c_ptr), value :: ens
+     end subroutine sd2d_ens_destroy
+
+     type(c_ptr) function sd2d_ens_last_error(ens) bind(C, name="sd2d_ens_last_error")
+       import :: c_ptr
+       type(c_ptr), value :: ens
+     end function sd2d_ens_last_error
+
+     ! member: 0-based index
+     integer(c_int) function sd2d_ens_set_state(ens, member, q, maxima) &
+          bind(C, name="sd2d_ens_set_state")
+       import :: c_int, c_int32_t, c_ptr, c_double, sd2d_maxima_t
+       type(c_ptr), value :: ens
+       integer(c_int32_t), value :: member
+       real(c_double), intent(in) :: q(*)
+       type(sd2d_maxima_t), intent(in), optional :: maxima
+     end function sd2d_ens_set_state
+
+     integer(c_int) function sd2d_ens_get_state(ens, member, q, maxima) &
+          bind(C, name="sd2d_ens_get_state")
+       import :: c_int, c_int32_t, c_ptr, c_double, sd2d_maxima_t
+       type(c_ptr), value :: ens
+       integer(c_int32_t), value :: member
+       real(c_double), intent(out), optional :: q(*)
+       type(sd2d_maxima_t), intent(out), optional :: maxima
+     end function sd2d_ens_get_state
+
+     integer(c_int) function sd2d_ens_set_time(ens, member, ctime, iter) &
+          bind(C, name="sd2d_ens_set_time")
+       import :: c_int, c_int32_t, c_ptr, c_double
+       type(c_ptr), value :: ens
+       integer(c_int32_t), value :: member
+       real(c_double), value :: ctime
+       integer(c_int32_t), value :: iter
+     end function sd2d_ens_set_time
+
+     integer(c_int) function sd2d_ens_get_time(ens, member, ctime, iter) &
+          bind(C, name="sd2d_ens_get_time")
+       import :: c_int, c_int32_t, c_ptr, c_double
+       type(c_ptr), value :: ens
+       integer(c_int32_t), value :: member
+       real(c_double), intent(out), optional :: ctime
+       integer(c_int32_t), intent(out), optional :: iter
+     end function sd2d_ens_get_time
+
+     integer(c_int) function sd2d_ens_step(ens, nsteps) bind(C, name="sd2d_ens_step")
+       import :: c_int, c_int32_t, c_ptr
+       type(c_ptr), value :: ens
+       integer(c_int32_t), value :: nsteps
+     end function sd2d_ens_step
+
+     ! shapes(3): the chosen SD2D_TILE_* for N = 3, 4, 5
+     integer(c_int) function sd2d_ens_tune(ens, trial_steps, shapes) bind(C, name="sd2d_ens_tune")
+       import :: c_int, c_int32_t, c_ptr
+       type(c_ptr), value :: ens
+       integer(c_int32_t), value :: trial_steps
+       integer(c_int32_t), intent(inout), optional :: shapes(*)
+     end function sd2d_ens_tune
+
+     ! adjoint: forward with checkpoints, then the vector-Jacobian product
+     integer(c_int) function sd2d_fwd(ctx, nsteps, ckpt, objective) bind(C, name="sd2d_fwd")
+       import :: c_int, c_int32_t, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: nsteps, ckpt
+       real(c_double), intent(out), optional :: objective
+     end function sd2d_fwd
+
+     integer(c_int) function sd2d_vjp(ctx, qbar_final, jbar, qbar_init, pbar, chbar) &
+          bind(C, name="sd2d_vjp")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(in), optional :: qbar_final(*)
+       real(c_double), value :: jbar
+       real(c_double), intent(out), optional :: qbar_init(*), pbar(*), chbar(*)
+     end function sd2d_vjp
+
+     integer(c_int) function sd2d_compresid_vjp(ctx, rbar, qbar, chbar, pbar) &
+          bind(C, name="sd2d_compresid_vjp")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(in) :: rbar(*)
+       real(c_double), intent(out), optional :: qbar(*)
+       real(c_double), intent(out), optional :: chbar
+       real(c_double), intent(out), optional :: pbar(*)
+     end function sd2d_compresid_vjp
   end interface
 
 contains

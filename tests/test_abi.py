@@ -237,3 +237,12 @@ def test_ensemble_rejects_bad_input():
     assert rc == abi.SD2D_EINVAL and "precision" in msg
     rc, _, msg = _ens_create(cs, lambda m, p: setattr(m[0], "N", 7))
     assert rc == abi.SD2D_EUNSUPPORTED and "N must be" in msg
+
+
+def test_fortran_module_declares_every_entry_point():
+    """sddc_amd/fortran/sd2d_c.f90 binds every function include/sd2d.h
+    declares, under its C name."""
+    import re
+    src = open(os.path.join(os.path.dirname(HERE), "sddc_amd", "fortran", "sd2d_c.f90")).read()
+    bound = sorted(set(re.findall(r'bind\(C,\s*name="(sd2d_\w+)"\)', src)))
+    assert bound == declared_functions()
