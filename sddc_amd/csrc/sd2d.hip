@@ -57,6 +57,7 @@ struct sd2d_ctx {
   // valid-cell count, per-member dt); 1 and null otherwise.  phm holds the
   // free stream of every member (one entry for one run).
   int nmem = 1;
+  long long nlaunch = 0;                // stage launches of sd2d_step so far
   int *tinfo = nullptr;
   double *dtm = nullptr;
   Phys *phm = nullptr;
@@ -608,6 +609,7 @@ static int step_rk(sd2d_ctx *c, bool norms) {
   const int av = (c->p.artiv == 1 && c->p.vismode == 1) ? M_AV : 0;
   auto launch = [&](StageArgs &a, int mode) -> int {
     if (mode & M_NORMS) a.normbuf = c->normbuf;
+    a.rev = c->nlaunch++ & 1;   // alternate the tile order stage by stage
     if (c->vis) {
       HIPCHK(c, launch_vis(c, a.qin));
       a.visres = c->visres;

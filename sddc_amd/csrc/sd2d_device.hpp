@@ -61,9 +61,17 @@ __host__ __device__ inline int finfo_pack(int kn, int left, int bc, int sneg) {
 
 constexpr double GAM = 5.0 / 3;   // setup2d.f90:67
 
-// 1/x: the correctly rounded quotient in FP64; in FP32 (the mixed-precision
-// path) the hardware reciprocal, accurate to 1 ulp.
-__device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
+// 1/x.  FP64: the hardware estimate refined by two Newton-Raphson steps
+// r <- r + r (1 - x r), within an ulp of the quotient (the division
+// sequence costs twice the instructions); FP32 (the mixed-precision path):
+// the hardware reciprocal, accurate to 1 ulp.
+__device__ __forceinline__ double rcp(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
 __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
 // Gas pressure (gamma-1)*(E - |m|^2/(2 rho) - |B|^2/2 - psi^2/2), the

@@ -60,6 +60,7 @@ struct StageArgs {
   // one entry per member
   const int *tinfo;
   const double *dtm;
+  int rev;                  // walk the launch order backwards (alternate stages)
   const Phys *phm;          // free stream and GLM alpha, one per member
   int ncell, ntile;
 };
@@ -74,10 +75,15 @@ enum { M_RK = 1, M_AV = 2, M_NORMS = 4, M_COMPAT = 8, M_AUX1 = 16, M_AUX4 = 32,
 
 // XCD-aware block order: consecutive positions of the tile launch order go
 // to the same XCD (and its L2), so the neighbour lines a tile reads were
-// staged by a nearby tile on the same XCD.  Bijective for any tile count.
-__device__ __forceinline__ int tile_of_block(int b, int nt) {
+// staged by a nearby tile on the same XCD.  With rev set, every XCD walks
+// its part of the order backwards: alternate stages then start where the
+// previous stage ended, on the tiles it wrote last (still in the
+// memory-side cache).  Bijective for any tile count either way.
+__device__ __forceinline__ int tile_of_block(int b, int nt, int rev) {
   const int q = nt >> 3, r = nt & 7;
-  const int x = b & 7, i = b >> 3;
+  const int x = b & 7;
+  int i = b >> 3;
+  if (rev) i = q + (x < r ? 1 : 0) - 1 - i;
   return (x < r) ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
 }
 
@@ -269,19 +275,21 @@ __global__ __launch_bounds__(STAGE_THREADS, 2) void stage_kernel(StageArgs a, Op
   constexpr bool COMPAT = (MODE & M_COMPAT) != 0, RK = (MODE & M_RK) != 0;
   constexpr int NAUX = (MODE & M_AUX4) ? 4 : ((MODE & M_AUX1) ? 1 : 0);
   constexpr bool RSTORE = (MODE & M_RSTORE) != 0, VIS = (MODE & M_VIS) != 0;
-  static_assert(NT <= STAGE_THREADS, "one thread per grid line");
+  static_assert(N * T <= 64 && STAGE_THREADS == 128, "one wavefront per line family");
   extern __shared__ double lds[];
   const int tid = threadIdx.x;
-  const int tile = a.torder[tile_of_block(blockIdx.x, a.ntile)];
-  const int ci = tid % T;
-  const int fl = tid / T;              // 0..2N-1 for busy threads
-  const int fam = fl >= N ? 1 : 0;     // 0: xi-line (row js), 1: eta-line (column is)
-  const int line = fl - fam * N;
+  const int tile = a.torder[tile_of_block(blockIdx.x, a.ntile, a.rev)];
+  // one family per wavefront (wave 0: xi-lines, wave 1: eta-lines), so the
+  // branches that depend on the family (in-tile or gathered neighbour
+  // traces, the points step) do not diverge inside a wave
+  const int fam = tid >> 6;            // 0: xi-line (row js), 1: eta-line (column is)
+  const int line = (tid & 63) / T;     // N lines of T cells; the rest of the wave idles
+  const int ci = (tid & 63) % T;
   const int cell = tile * T + ci;
   // ensemble: the tile's member and valid-cell count; one run: member 0
   int mem = 0, nvt = a.ncell - tile * T;
   if (a.tinfo) { const int ti = a.tinfo[tile]; mem = ti >> 6; nvt = ti & 63; }
-  const bool valid = tid < NT && ci < nvt;
+  const bool valid = line < N && ci < nvt;
   const double ge = a.gmax[2 * mem], gu = a.gmax[2 * mem + 1];
   const double cr = a.dtm ? a.cr * a.dtm[mem] : a.cr;   // b * dt of the member
   const Phys &ph = a.phm[mem];                            // the member's free stream
@@ -516,7 +524,7 @@ __global__ __launch_bounds__(STAGE_THREADS, 2) void stage_kernel(StageArgs a, Op
           R[v] = -D[s][v] * ji;
       }
       const double divB = db[s] * ji;
-      const double ri = 1.0 / q[0];
+      const double ri = rcp(q[0]);
       const double ux = q[1] * ri, uy = q[2] * ri, uz = q[3] * ri;
       // Powell source and GLM damping, compresid.f90:58-74
       R[1] -= divB * q[5];

@@ -65,20 +65,22 @@ void stage_kernel_mx(StageArgs a, OpsF ops, const float *__restrict__ metf) {
   constexpr int NAUX = (MODE & M_AUX4) ? 4 : ((MODE & M_AUX1) ? 1 : 0);
   constexpr bool RSTORE = (MODE & M_RSTORE) != 0;
   static_assert(RK && (MODE & M_COMPAT) == 0, "the mixed kernel is the stepping kernel");
-  static_assert(NT <= STAGE_THREADS, "one thread per grid line");
+  static_assert(N * T <= 64 && STAGE_THREADS == 128, "one wavefront per line family");
   extern __shared__ double lds[];
   float *ldf = reinterpret_cast<float *>(lds);
   const int tid = threadIdx.x;
-  const int tile = a.torder[tile_of_block(blockIdx.x, a.ntile)];
-  const int ci = tid % T;
-  const int fl = tid / T;
-  const int fam = fl >= N ? 1 : 0;
-  const int line = fl - fam * N;
+  const int tile = a.torder[tile_of_block(blockIdx.x, a.ntile, a.rev)];
+  // one family per wavefront (wave 0: xi-lines, wave 1: eta-lines), so the
+  // branches that depend on the family (in-tile or gathered neighbour
+  // traces, the points step) do not diverge inside a wave
+  const int fam = tid >> 6;            // 0: xi-line (row js), 1: eta-line (column is)
+  const int line = (tid & 63) / T;     // N lines of T cells; the rest of the wave idles
+  const int ci = (tid & 63) % T;
   const int cell = tile * T + ci;
   // ensemble: the tile's member and valid-cell count; one run: member 0
   int mem = 0, nvt = a.ncell - tile * T;
   if (a.tinfo) { const int ti = a.tinfo[tile]; mem = ti >> 6; nvt = ti & 63; }
-  const bool valid = tid < NT && ci < nvt;
+  const bool valid = line < N && ci < nvt;
   const double ge = a.gmax[2 * mem], gu = a.gmax[2 * mem + 1];
   const double cr = a.dtm ? a.cr * a.dtm[mem] : a.cr;   // b * dt of the member
   const Phys &ph = a.phm[mem];                            // the member's free stream
