@@ -170,15 +170,26 @@ __device__ __forceinline__ void neighbour_line(int kn, int nfp, int &p0, int &st
 // own trace: interfaceflux_all.f90:59-76).  When the neighbour cell is in
 // the same tile, nothing is read here: lci returns its index in the tile
 // and the trace is formed from the staged tile by face_trace_lds.
-template <int N, typename R, typename OP>
-__device__ __forceinline__ int face_gather(const StageArgs &a, const OP &ops, int tile, int ci,
-                                           int fam, int line, int rfp, R *Qn, int &lci) {
-  constexpr int T = Geo<N>::CPW, N2 = Geo<N>::N2, QT = Geo<N>::QT;
+// The face-table words (neighbour base, info) of end face `rfp` of a line.
+template <int N>
+__device__ __forceinline__ void face_words(const StageArgs &a, int tile, int ci, int fam, int line,
+                                           int rfp, int &nbase, int &info) {
+  constexpr int T = Geo<N>::CPW;
   int k, nfp;
   end_face<N>(fam, line, rfp, k, nfp);
   const int *ft = a.ftab + ((size_t)tile * 8 + k * 2) * T + ci;
-  const int nbase = ft[0];
-  const int info = ft[T];
+  nbase = ft[0];
+  info = ft[T];
+}
+
+// The gather of face_gather from already loaded face-table words.
+template <int N, typename R, typename OP>
+__device__ __forceinline__ void face_gather_w(const StageArgs &a, const OP &ops, int tile, int fam,
+                                              int line, int rfp, int nbase, int info, R *Qn,
+                                              int &lci) {
+  constexpr int T = Geo<N>::CPW, N2 = Geo<N>::N2, QT = Geo<N>::QT;
+  int k, nfp;
+  end_face<N>(fam, line, rfp, k, nfp);
   lci = -1;
   if (((info >> 3) & 7) == FT_INTERIOR) {
     const int ntile = nbase / QT;
@@ -200,6 +211,20 @@ __device__ __forceinline__ int face_gather(const StageArgs &a, const OP &ops, in
       }
     }
   }
+}
+
+// Gather for an end face: the face-table info word and, for an interior or
+// periodic face, the neighbour's trace Qn at the shared face point (its
+// line through that point, interpolated with the same arithmetic as the
+// own trace: interfaceflux_all.f90:59-76).  When the neighbour cell is in
+// the same tile, nothing is read here: lci returns its index in the tile
+// and the trace is formed from the staged tile by face_trace_lds.
+template <int N, typename R, typename OP>
+__device__ __forceinline__ int face_gather(const StageArgs &a, const OP &ops, int tile, int ci,
+                                           int fam, int line, int rfp, R *Qn, int &lci) {
+  int nbase, info;
+  face_words<N>(a, tile, ci, fam, line, rfp, nbase, info);
+  face_gather_w<N, R>(a, ops, tile, fam, line, rfp, nbase, info, Qn, lci);
   return info;
 }
 
@@ -301,6 +326,13 @@ __global__ __launch_bounds__(STAGE_THREADS, 2) void stage_kernel(StageArgs a, Op
   const int mx = fam ? G::S2X : G::S1X, my = fam ? G::S2Y : G::S1Y;
   const double *ql = lds + LD::Q0 + p0 * T + ci;   // line values: ql[(v*N2 + ps*s)*T]
 
+  // ---- the face-table words of the line's two end faces first: the
+  //      neighbour gathers depend on them -----------------------------------------
+  int nb0 = 0, nbN = 0, info0 = 0, infoN = 0;
+  if (valid) {
+    face_words<N>(a, tile, ci, fam, line, 0, nb0, info0);
+    face_words<N>(a, tile, ci, fam, line, N, nbN, infoN);
+  }
   // ---- the tile's state -> LDS in 16-byte loads (one contiguous block; all
   //      loads are issued before the first LDS store) -------------------------
   {
@@ -325,10 +357,20 @@ __global__ __launch_bounds__(STAGE_THREADS, 2) void stage_kernel(StageArgs a, Op
   //      neighbour in another tile, formed from the staged tile after the
   //      barrier for a neighbour in this one --------------------------------------
   double qn0[9], qnN[9];
-  int info0 = 0, infoN = 0, lci0 = -1, lciN = -1;
+  int lci0 = -1, lciN = -1;
   if (valid) {
-    info0 = face_gather<N, double>(a, ops, tile, ci, fam, line, 0, qn0, lci0);
-    infoN = face_gather<N, double>(a, ops, tile, ci, fam, line, N, qnN, lciN);
+    face_gather_w<N, double>(a, ops, tile, fam, line, 0, nb0, info0, qn0, lci0);
+    face_gather_w<N, double>(a, ops, tile, fam, line, N, nbN, infoN, qnN, lciN);
+  }
+  // the metric vectors of the line's N+1 flux points, loaded while the
+  // gathers are in flight
+  double msx[NF], msy[NF];
+  if (valid) {
+#pragma unroll
+    for (int r = 0; r < NF; ++r) {
+      msx[r] = mt[(mx + r + NF * line) * T];
+      msy[r] = mt[(my + r + NF * line) * T];
+    }
   }
   __syncthreads();
   if (valid && lci0 >= 0) face_trace_lds<N, double>(lds + LD::Q0, ops, info0, fam, line, 0, lci0, qn0);
@@ -349,7 +391,7 @@ __global__ __launch_bounds__(STAGE_THREADS, 2) void stage_kernel(StageArgs a, Op
         for (int s = 0; s < N; ++s) vals[s] = ql[(v * N2 + ps * s) * T];
         t0[v] = line_trace<N, double>(vals, ops.L[0]);
       }
-      const double sx = mt[(mx + NF * line) * T], sy = mt[(my + NF * line) * T];
+      const double sx = msx[0], sy = msy[0];
       double Qv[9];
       face_flux<N, double>(info0, fam, 0, ph, c_h, t0, qn0, sx, sy, F0, Qv, eig, um);
       // gradient weights of the averaged field (nablaQsp, compflux.f90:107-157:
@@ -371,7 +413,7 @@ __global__ __launch_bounds__(STAGE_THREADS, 2) void stage_kernel(StageArgs a, Op
         for (int s = 0; s < N; ++s) vals[s] = ql[(v * N2 + ps * s) * T];
         tN[v] = line_trace<N, double>(vals, ops.L[N]);
       }
-      const double sx = mt[(mx + N + NF * line) * T], sy = mt[(my + N + NF * line) * T];
+      const double sx = msx[N], sy = msy[N];
       double FN[9], Qv[9];
       face_flux<N, double>(infoN, fam, N, ph, c_h, tN, qnN, sx, sy, FN, Qv, eig, um);
       const double wbN = Qv[5] * sx + Qv[6] * sy;
@@ -401,7 +443,7 @@ __global__ __launch_bounds__(STAGE_THREADS, 2) void stage_kernel(StageArgs a, Op
         for (int s = 0; s < N; ++s) vals[s] = ql[(v * N2 + ps * s) * T];
         tr[v] = line_trace<N, double>(vals, ops.L[r]);
       }
-      const double sx = mt[(mx + r + NF * line) * T], sy = mt[(my + r + NF * line) * T];
+      const double sx = msx[r], sy = msy[r];
       double F[9];
       iflux_n<double>(tr, c_h, sx, sy, F, um);
       const double wb = tr[5] * sx + tr[6] * sy;
