@@ -1,6 +1,6 @@
 // sd2d_stage.hpp - the fused residual + Runge-Kutta stage kernel.
 //
-// One 128-thread workgroup owns a tile of T cells (T = 12/16/21 for
+// One 128-thread workgroup owns a tile of T cells (T = 12/16/20 for
 // N = 5/4/3) and runs one thread per grid line: thread (fam, line, cell)
 // takes the N solution values of its line for all 9 variables and, in
 // registers,
@@ -12,9 +12,10 @@
 //   - differentiates the fluxes along the line (Mmat), and the normal
 //     component of the averaged field (divB) and its rotated counterpart
 //     (Jz, for the AV sensor) likewise.
-// The eta-line threads (fam 1) hand their derivatives to the xi-line
-// threads (fam 0) through LDS, in two rounds; each xi-line thread then finishes the
-// N solution points of its line: residual, Powell and GLM sources, the RK
+// The two families exchange their derivatives through LDS in two rounds
+// and split the solution points' work by variable: each xi-line thread
+// (fam 0) finishes variables 0-4, each eta-line thread (fam 1) variables
+// 5-8 at the N points of its line: residual, Powell and GLM sources, the RK
 // combination, norm partials.  The running maxima and norms are reduced
 // per workgroup; the AV smoothness sensor runs per cell on the last stage.
 //
@@ -29,6 +30,8 @@
 // both cells store bitwise the same flux and only the state, the metrics
 // and the RK registers move through HBM.
 #pragma once
+#include <type_traits>
+
 #include "sd2d_device.hpp"
 
 namespace sd2d {
@@ -88,10 +91,11 @@ __device__ __forceinline__ int tile_of_block(int b, int nt, int rev) {
 }
 
 // LDS of one workgroup (in doubles): the tile's state [v][p][ci], the
-// eta-line derivatives of the fluxes of XV variables at a time [v][p][ci]
-// (two rounds cover the nine), of the divB weight [p][ci] and (AV) of the
-// Jz weight [p][ci], then reduction slots.  The AV sensor reuses the state
-// area once the residual is formed.
+// derivatives of the fluxes of up to XV variables [v][p][ci] handed from one
+// family to the other (eta -> xi: variables 0-4, then xi -> eta: 5-8), of
+// the divB weight [p][ci] and (AV) of the Jz weight [p][ci], then
+// reduction slots.  The AV sensor reuses the state area once the residual
+// is formed.
 template <int N, int MODE> struct StageLds {
   static constexpr int T = Geo<N>::CPW, N2 = N * N, NL = N - 1;
   static constexpr bool AV = (MODE & M_AV) != 0;
@@ -459,55 +463,72 @@ __global__ __launch_bounds__(STAGE_THREADS, 2) void stage_kernel(StageArgs a, Op
     __builtin_amdgcn_sched_barrier(0);
   }
 
-  // ---- RK register of the stage update at the xi-line's points, loaded now
-  //      so that its latency hides behind the exchange below (the output may
-  //      be the same array, so these loads cannot move past its stores) -------
-  //      (the first NPF points; each later point is loaded NPF points ahead
-  //      of its use, before the stores of the point in hand)
-  constexpr int NPF = (RK && NAUX == 1) ? ((AV || NORMS) ? 2 : 3) : 1;
-  double ax[NPF][9];
+  // ---- the two families share the solution points' work by variable: the
+  //      xi-line threads finish variables 0..VS-1 at their N points, the
+  //      eta-line threads variables VS..8 at theirs, so both waves of the
+  //      workgroup carry half of the point updates.  The RK register of the
+  //      variables a thread finishes is loaded now, so that its latency hides
+  //      behind the exchange (the output may be the same array, so these
+  //      loads cannot move past its stores) -------------------------------------
+  constexpr int VS = 5;
+  static_assert(VS <= LD::XV && 9 - VS <= LD::XV, "exchange area holds one round");
+  double ax[N][VS];
   const double *aux0 = a.aux[0] + (size_t)tile * G::QT + ci;
-  if (RK && NAUX == 1 && valid && fam == 0) {
+  if (RK && NAUX == 1 && valid) {
+    if (fam == 0) {
 #pragma unroll
-    for (int s = 0; s < NPF; ++s)
+      for (int s = 0; s < N; ++s)
 #pragma unroll
-      for (int v = 0; v < 9; ++v) ax[s][v] = aux0[(v * N2 + N * line + s) * T];
+        for (int v = 0; v < VS; ++v) ax[s][v] = aux0[(v * N2 + p0 + ps * s) * T];
+    } else {
+#pragma unroll
+      for (int s = 0; s < N; ++s)
+#pragma unroll
+        for (int v = VS; v < 9; ++v) ax[s][v - VS] = aux0[(v * N2 + p0 + ps * s) * T];
+    }
   }
 
-  // ---- the eta-line threads hand their derivatives to the xi-line threads,
-  //      XV variables per round; afterwards D, db, dj of a xi-line thread hold
-  //      the sum of both families at its points ---------------------------------
-  constexpr int XV = LD::XV;
+  // ---- exchange of the derivatives through LDS.  Round A: the eta-line
+  //      threads hand variables 0..VS-1 and the divB (Jz) weights to the
+  //      xi-line threads.  Round B: the xi-line threads hand variables VS..8 and
+  //      the summed divB weight to the eta-line threads.  Each LDS location is
+  //      read and then rewritten by the same xi-line lane, so one area serves
+  //      both rounds.  Afterwards D of each thread holds the sum of both
+  //      families for the variables it finishes, db the summed divB weight,
+  //      dj (xi-lines, AV) the summed Jz weight -------------------------------------
+  if (valid && fam == 1) {
 #pragma unroll
-  for (int r0 = 0; r0 < 9; r0 += XV) {
-    if (valid && fam == 1) {
+    for (int s = 0; s < N; ++s) {
+      const int p = p0 + ps * s;
 #pragma unroll
-      for (int s = 0; s < N; ++s) {
-        const int p = p0 + ps * s;
-#pragma unroll
-        for (int v = r0; v < r0 + XV && v < 9; ++v)
-          lds[LD::XD + ((v - r0) * N2 + p) * T + ci] = D[s][v];
-        if (r0 == 0) {
-          lds[LD::XB + p * T + ci] = db[s];
-          if (AV) lds[LD::XJ + p * T + ci] = dj[s];
-        }
-      }
+      for (int v = 0; v < VS; ++v) lds[LD::XD + (v * N2 + p) * T + ci] = D[s][v];
+      lds[LD::XB + p * T + ci] = db[s];
+      if (AV) lds[LD::XJ + p * T + ci] = dj[s];
     }
-    __syncthreads();
-    if (valid && fam == 0) {
+  }
+  __syncthreads();
+  if (valid && fam == 0) {
 #pragma unroll
-      for (int s = 0; s < N; ++s) {
-        const int p = N * line + s;
+    for (int s = 0; s < N; ++s) {
+      const int p = N * line + s;
 #pragma unroll
-        for (int v = r0; v < r0 + XV && v < 9; ++v)
-          D[s][v] += lds[LD::XD + ((v - r0) * N2 + p) * T + ci];
-        if (r0 == 0) {
-          db[s] += lds[LD::XB + p * T + ci];
-          if (AV) dj[s] += lds[LD::XJ + p * T + ci];
-        }
-      }
+      for (int v = 0; v < VS; ++v) D[s][v] += lds[LD::XD + (v * N2 + p) * T + ci];
+      db[s] += lds[LD::XB + p * T + ci];
+      if (AV) dj[s] += lds[LD::XJ + p * T + ci];
+#pragma unroll
+      for (int v = VS; v < 9; ++v) lds[LD::XD + ((v - VS) * N2 + p) * T + ci] = D[s][v];
+      lds[LD::XB + p * T + ci] = db[s];
     }
-    if (r0 + XV < 9) __syncthreads();
+  }
+  __syncthreads();
+  if (valid && fam == 1) {
+#pragma unroll
+    for (int s = 0; s < N; ++s) {
+      const int p = p0 + ps * s;
+#pragma unroll
+      for (int v = VS; v < 9; ++v) D[s][v] = lds[LD::XD + ((v - VS) * N2 + p) * T + ci] + D[s][v];
+      db[s] = lds[LD::XB + p * T + ci];
+    }
   }
 
   // all-variable gradient nablaQs for the compresid output: the xi-line
@@ -541,23 +562,26 @@ __global__ __launch_bounds__(STAGE_THREADS, 2) void stage_kernel(StageArgs a, Op
       }
     }
   };
-  if (COMPAT) {
-    if (valid && fam == 0) compat_gradient(false);
-    __syncthreads();
-    if (valid && fam == 1) compat_gradient(true);
-  }
-
-  // ---- solution points of the xi-lines: point (is, js) = (s, line) ---------
+  // ---- solution points: point (is, js) = (s, line) of a xi-line, (line, s)
+  //      of an eta-line; variables V0..V1-1 of it -----------------------------------
   double rho_s[N], jz2_s[N];
-  if (valid && fam == 0) {
+  auto points = [&](auto v0c, auto v1c) {
+    constexpr int V0 = decltype(v0c)::value, V1 = decltype(v1c)::value;
 #pragma unroll
     for (int s = 0; s < N; ++s) {
-      const int p = N * line + s;
-      const double ji = mt[(G::JI + line + N * s) * T];
+      const int p = p0 + ps * s;
+      const int is = fam ? line : s, js = fam ? s : line;
+      const double ji = mt[(G::JI + js + N * is) * T];
       double R[9], q[9];
 #pragma unroll
       for (int v = 0; v < 9; ++v) {
-        q[v] = lds[LD::Q0 + (v * N2 + p) * T + ci];
+        // the velocity and the field enter the sources of every variable;
+        // the energy and psi only their own update
+        if (v < 4 || (v >= 5 && v < 8) || (v >= V0 && v < V1))
+          q[v] = lds[LD::Q0 + (v * N2 + p) * T + ci];
+      }
+#pragma unroll
+      for (int v = V0; v < V1; ++v) {
         // resid = -(dF/dxi + dG/deta) / J   (compresid.f90:41,56), with the
         // viscous divergence added before the scaling (compresid.f90:43-46)
         if (VIS)
@@ -569,60 +593,50 @@ __global__ __launch_bounds__(STAGE_THREADS, 2) void stage_kernel(StageArgs a, Op
       const double ri = rcp(q[0]);
       const double ux = q[1] * ri, uy = q[2] * ri, uz = q[3] * ri;
       // Powell source and GLM damping, compresid.f90:58-74
-      R[1] -= divB * q[5];
-      R[2] -= divB * q[6];
-      R[3] -= divB * q[7];
-      R[4] -= divB * (ux * q[5] + uy * q[6] + uz * q[7]);
-      R[5] -= divB * ux;
-      R[6] -= divB * uy;
-      R[7] -= divB * uz;
-      R[8] -= ph.alpha * q[8];
-      if (AV) {
+      if (V0 <= 1 && 1 < V1) R[1] -= divB * q[5];
+      if (V0 <= 2 && 2 < V1) R[2] -= divB * q[6];
+      if (V0 <= 3 && 3 < V1) R[3] -= divB * q[7];
+      if (V0 <= 4 && 4 < V1) R[4] -= divB * (ux * q[5] + uy * q[6] + uz * q[7]);
+      if (V0 <= 5 && 5 < V1) R[5] -= divB * ux;
+      if (V0 <= 6 && 6 < V1) R[6] -= divB * uy;
+      if (V0 <= 7 && 7 < V1) R[7] -= divB * uz;
+      if (V0 <= 8 && 8 < V1) R[8] -= ph.alpha * q[8];
+      if (AV && V0 == 0) {
         const double jz = dj[s] * ji;
         rho_s[s] = q[0];
         jz2_s[s] = jz * jz;
       }
       if (NORMS) {
 #pragma unroll
-        for (int v = 0; v < 9; ++v) nres = fma(R[v], R[v], nres);
-        ndiv = fma(divB, divB, ndiv);
+        for (int v = V0; v < V1; ++v) nres = fma(R[v], R[v], nres);
+        if (V0 == 0) ndiv = fma(divB, divB, ndiv);
       }
       if (COMPAT) {
         const size_t pf = (size_t)p + (size_t)N2 * a.s2c[cell];
 #pragma unroll
-        for (int v = 0; v < 9; ++v) a.resid_f[v + 9 * pf] = R[v];
+        for (int v = V0; v < V1; ++v) a.resid_f[v + 9 * pf] = R[v];
       }
       if (RK) {
         const size_t off = (size_t)tile * G::QT + ci;
-        // all RK-register values of the point are read before its stores
         // several registers (FiveStageRk's last stage): their weighted sum,
-        // formed before the point's stores
+        // formed before the point's stores (the output may be one of them)
         double xs[9];
         if (NAUX > 1) {
 #pragma unroll
-          for (int v = 0; v < 9; ++v) xs[v] = 0.0;
+          for (int v = V0; v < V1; ++v) xs[v] = 0.0;
 #pragma unroll
           for (int i = 0; i < NAUX; ++i) {
 #pragma unroll
-            for (int v = 0; v < 9; ++v)
+            for (int v = V0; v < V1; ++v)
               xs[v] = fma(a.ca[i], a.aux[i][off + (size_t)(v * N2 + p) * T], xs[v]);
-            __builtin_amdgcn_sched_barrier(0);   // one register's 9 loads in flight at a time
-          }
-        }
-        double xc[9];
-        if (NAUX == 1) {
-#pragma unroll
-          for (int v = 0; v < 9; ++v) xc[v] = ax[s % NPF][v];
-          if (s + NPF < N) {
-#pragma unroll
-            for (int v = 0; v < 9; ++v) ax[s % NPF][v] = aux0[(v * N2 + p + NPF) * T];
+            __builtin_amdgcn_sched_barrier(0);   // one register's loads in flight at a time
           }
         }
 #pragma unroll
-        for (int v = 0; v < 9; ++v) {
+        for (int v = V0; v < V1; ++v) {
           const size_t o = off + (size_t)(v * N2 + p) * T;
           double acc = a.cin * q[v];
-          if (NAUX == 1) acc = fma(a.ca[0], xc[v], acc);
+          if (NAUX == 1) acc = fma(a.ca[0], ax[s][v - V0], acc);
           if (NAUX > 1) acc += xs[v];
           acc = fma(cr, R[v], acc);
           a.qout[o] = acc;
@@ -630,6 +644,16 @@ __global__ __launch_bounds__(STAGE_THREADS, 2) void stage_kernel(StageArgs a, Op
         }
       }
     }
+  };
+  if (valid) {
+    if (fam == 0) points(std::integral_constant<int, 0>{}, std::integral_constant<int, VS>{});
+    else points(std::integral_constant<int, VS>{}, std::integral_constant<int, 9>{});
+  }
+  // (after the points, when the derivative accumulators are free)
+  if (COMPAT) {
+    if (valid && fam == 0) compat_gradient(false);
+    __syncthreads();
+    if (valid && fam == 1) compat_gradient(true);
   }
 
   // ---- block reductions: running maxima (getrusanov.f90:116-123,188-191) and
