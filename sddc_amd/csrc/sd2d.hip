@@ -457,7 +457,7 @@ int sd2d_create(const sd2d_mesh *m, const sd2d_params *p, sd2d_ctx **out) {
     const int r = build_host_tables(m, p, tb, e);
     if (r) { delete c; return fail(nullptr, r, e); }
   }
-  const int N = m->N, nc = m->ncell, CPW = tb.cpw, N2 = N * N;
+  const int N = m->N, nc = m->ncell, CPW = tb.cpw;
   c->N = N; c->ncell = nc; c->cpw = CPW; c->ntile = tb.ntile; c->p = *p;
   c->ctime = 0.0; c->iter = 0;
   c->ops = tb.ops;

@@ -299,7 +299,7 @@ template <int N, int MODE>
 __global__ __launch_bounds__(STAGE_THREADS, 2) void stage_kernel(StageArgs a, Ops ops) {
   using G = Geo<N>;
   using LD = StageLds<N, MODE>;
-  constexpr int T = G::CPW, N2 = G::N2, NF = N + 1, NT = 2 * N * T;
+  constexpr int T = G::CPW, N2 = G::N2, NF = N + 1;
   constexpr bool AV = (MODE & M_AV) != 0, NORMS = (MODE & M_NORMS) != 0;
   constexpr bool COMPAT = (MODE & M_COMPAT) != 0, RK = (MODE & M_RK) != 0;
   constexpr int NAUX = (MODE & M_AUX4) ? 4 : ((MODE & M_AUX1) ? 1 : 0);

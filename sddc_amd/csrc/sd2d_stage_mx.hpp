@@ -13,7 +13,8 @@
 //          combination, the norms and the AV sensor.
 // The tile's state is staged in LDS as FP32 for the traces; the points step
 // reads the FP64 state from memory (one point ahead of its use), and the
-// FP32 staging area is reused for the exchange of the eta-line derivatives.
+// FP32 staging area is reused for the exchange of the derivatives between
+// the two line families, which split the points' work by variable.
 // About 1.1 KB of LDS per cell and roughly half the registers of the FP64
 // kernel, so more workgroups share a CU.
 #pragma once
@@ -59,7 +60,7 @@ __global__ __launch_bounds__(STAGE_THREADS,
 void stage_kernel_mx(StageArgs a, OpsF ops, const float *__restrict__ metf) {
   using G = Geo<N>;
   using LD = StageLdsMx<N, MODE>;
-  constexpr int T = G::CPW, N2 = G::N2, NF = N + 1, NT = 2 * N * T;
+  constexpr int T = G::CPW, N2 = G::N2, NF = N + 1;
   constexpr bool AV = (MODE & M_AV) != 0, NORMS = (MODE & M_NORMS) != 0;
   constexpr bool RK = (MODE & M_RK) != 0;
   constexpr int NAUX = (MODE & M_AUX4) ? 4 : ((MODE & M_AUX1) ? 1 : 0);
@@ -76,7 +77,6 @@ void stage_kernel_mx(StageArgs a, OpsF ops, const float *__restrict__ metf) {
   const int fam = tid >> 6;            // 0: xi-line (row js), 1: eta-line (column is)
   const int line = (tid & 63) / T;     // N lines of T cells; the rest of the wave idles
   const int ci = (tid & 63) % T;
-  const int cell = tile * T + ci;
   // ensemble: the tile's member and valid-cell count; one run: member 0
   int mem = 0, nvt = a.ncell - tile * T;
   if (a.tinfo) { const int ti = a.tinfo[tile]; mem = ti >> 6; nvt = ti & 63; }
@@ -94,6 +94,13 @@ void stage_kernel_mx(StageArgs a, OpsF ops, const float *__restrict__ metf) {
   const int mx = fam ? G::S2X : G::S1X, my = fam ? G::S2Y : G::S1Y;
   const float *ql = ldf + LD::QF + p0 * T + ci;
 
+  // ---- the face-table words of the line's two end faces first: the
+  //      neighbour gathers depend on them -----------------------------------------
+  int nb0 = 0, nbN = 0, info0 = 0, infoN = 0;
+  if (valid) {
+    face_words<N>(a, tile, ci, fam, line, 0, nb0, info0);
+    face_words<N>(a, tile, ci, fam, line, N, nbN, infoN);
+  }
   // ---- the tile's state -> LDS as FP32 --------------------------------------
   {
     constexpr int NV2 = 9 * N2 * T / 2;
@@ -114,10 +121,20 @@ void stage_kernel_mx(StageArgs a, OpsF ops, const float *__restrict__ metf) {
   }
   // ---- neighbour traces at both end faces (memory or the staged tile) -------
   float qn0[9], qnN[9];
-  int info0 = 0, infoN = 0, lci0 = -1, lciN = -1;
+  int lci0 = -1, lciN = -1;
   if (valid) {
-    info0 = face_gather<N, float>(a, ops, tile, ci, fam, line, 0, qn0, lci0);
-    infoN = face_gather<N, float>(a, ops, tile, ci, fam, line, N, qnN, lciN);
+    face_gather_w<N, float>(a, ops, tile, fam, line, 0, nb0, info0, qn0, lci0);
+    face_gather_w<N, float>(a, ops, tile, fam, line, N, nbN, infoN, qnN, lciN);
+  }
+  // the metric vectors of the line's N+1 flux points, loaded while the
+  // gathers are in flight
+  float msx[NF], msy[NF];
+  if (valid) {
+#pragma unroll
+    for (int r = 0; r < NF; ++r) {
+      msx[r] = mtf[(mx + r + NF * line) * T];
+      msy[r] = mtf[(my + r + NF * line) * T];
+    }
   }
   __syncthreads();
   if (valid && lci0 >= 0) face_trace_lds<N, float>(ldf + LD::QF, ops, info0, fam, line, 0, lci0, qn0);
@@ -135,7 +152,7 @@ void stage_kernel_mx(StageArgs a, OpsF ops, const float *__restrict__ metf) {
         for (int s = 0; s < N; ++s) vals[s] = ql[(v * N2 + ps * s) * T];
         t0[v] = line_trace<N, float>(vals, ops.L[0]);
       }
-      const float sx = mtf[(mx + NF * line) * T], sy = mtf[(my + NF * line) * T];
+      const float sx = msx[0], sy = msy[0];
       float Qv[9];
       face_flux<N, float>(info0, fam, 0, ph, c_h, t0, qn0, sx, sy, F0, Qv, eig, um);
       wb0 = Qv[5] * sx + Qv[6] * sy;
@@ -151,7 +168,7 @@ void stage_kernel_mx(StageArgs a, OpsF ops, const float *__restrict__ metf) {
         for (int s = 0; s < N; ++s) vals[s] = ql[(v * N2 + ps * s) * T];
         tN[v] = line_trace<N, float>(vals, ops.L[N]);
       }
-      const float sx = mtf[(mx + N + NF * line) * T], sy = mtf[(my + N + NF * line) * T];
+      const float sx = msx[N], sy = msy[N];
       float FN[9], Qv[9];
       face_flux<N, float>(infoN, fam, N, ph, c_h, tN, qnN, sx, sy, FN, Qv, eig, um);
       const float wbN = Qv[5] * sx + Qv[6] * sy;
@@ -175,7 +192,7 @@ void stage_kernel_mx(StageArgs a, OpsF ops, const float *__restrict__ metf) {
         for (int s = 0; s < N; ++s) vals[s] = ql[(v * N2 + ps * s) * T];
         tr[v] = line_trace<N, float>(vals, ops.L[r]);
       }
-      const float sx = mtf[(mx + r + NF * line) * T], sy = mtf[(my + r + NF * line) * T];
+      const float sx = msx[r], sy = msy[r];
       float F[9];
       iflux_n<float>(tr, c_h, sx, sy, F, um);
       const float wb = tr[5] * sx + tr[6] * sy;
@@ -191,102 +208,144 @@ void stage_kernel_mx(StageArgs a, OpsF ops, const float *__restrict__ metf) {
     __builtin_amdgcn_sched_barrier(0);
   }
 
-  // ---- FP64 state and RK register at the xi-line's first point, loaded now;
-  //      every later point one point ahead of its use, before the stores of
-  //      the point in hand ----------------------------------------------------------
-  const double *qg = a.qin + (size_t)tile * G::QT + ci + N * line * T;
-  const double *aux0 = (NAUX == 1 ? a.aux[0] : a.qin) + (size_t)tile * G::QT + ci + N * line * T;
-  double qn[9], xn[9];
-  if (valid && fam == 0) {
+  // ---- the two families share the solution points' work by variable, as in
+  //      stage_kernel: xi-lines finish variables 0..VS-1, eta-lines VS..8.
+  //      The FP64 state and RK register of a thread's first point are loaded
+  //      now; every later point one point ahead of its use, before the
+  //      stores of the point in hand ------------------------------------------------
+  constexpr int VS = 5;
+  const double *qg = a.qin + (size_t)tile * G::QT + ci;
+  const double *aux0 = (NAUX == 1 ? a.aux[0] : a.qin) + (size_t)tile * G::QT + ci;
+  // the velocity and the field enter the sources of every variable; the
+  // energy and psi only their own update
+  auto needed = [](int v, int v0, int v1) { return v < 4 || (v >= 5 && v < 8) || (v >= v0 && v < v1); };
+  double qn[9], xn[VS];
+  if (valid) {
+    const int v0 = fam ? VS : 0, v1 = fam ? 9 : VS;
 #pragma unroll
-    for (int v = 0; v < 9; ++v) {
-      qn[v] = qg[v * N2 * T];
-      xn[v] = NAUX == 1 ? aux0[v * N2 * T] : 0.0;
-    }
+    for (int v = 0; v < 9; ++v)
+      if (needed(v, v0, v1)) qn[v] = qg[(v * N2 + p0) * T];
+#pragma unroll
+    for (int v = 0; v < VS; ++v)
+      xn[v] = (NAUX == 1 && v0 + v < v1) ? aux0[((v0 + v) * N2 + p0) * T] : 0.0;
   }
   __syncthreads();   // every thread is done with the staged state
 
-  // ---- the eta-line threads hand their derivatives to the xi-line threads ----
+  // ---- exchange through the former staging area (one slot per variable):
+  //      round A, eta -> xi: variables 0..VS-1 and the divB (Jz) weights;
+  //      round B, xi -> eta: variables VS..8 and the summed divB weight ---------
   if (valid && fam == 1) {
 #pragma unroll
     for (int s = 0; s < N; ++s) {
       const int p = p0 + ps * s;
 #pragma unroll
-      for (int v = 0; v < 9; ++v) ldf[LD::QF + (v * N2 + p) * T + ci] = D[s][v];
+      for (int v = 0; v < VS; ++v) ldf[LD::QF + (v * N2 + p) * T + ci] = D[s][v];
       ldf[LD::XB + p * T + ci] = db[s];
       if (AV) ldf[LD::XJ + p * T + ci] = dj[s];
     }
   }
   __syncthreads();
-
-  // ---- solution points of the xi-lines, in FP64 ---------------------------------
-  double rho_s[N], jz2_s[N];
   if (valid && fam == 0) {
 #pragma unroll
     for (int s = 0; s < N; ++s) {
       const int p = N * line + s;
-      const double ji = mt[(G::JI + line + N * s) * T];
-      double R[9], q[9], xc[9];
 #pragma unroll
-      for (int v = 0; v < 9; ++v) {
-        q[v] = qn[v];
-        xc[v] = xn[v];
-        // resid = -(dF/dxi + dG/deta) / J   (compresid.f90:41,56)
-        R[v] = -double(D[s][v] + ldf[LD::QF + (v * N2 + p) * T + ci]) * ji;
-      }
-      const double divB = double(db[s] + ldf[LD::XB + p * T + ci]) * ji;
+      for (int v = 0; v < VS; ++v) D[s][v] += ldf[LD::QF + (v * N2 + p) * T + ci];
+      db[s] += ldf[LD::XB + p * T + ci];
+      if (AV) dj[s] += ldf[LD::XJ + p * T + ci];
+#pragma unroll
+      for (int v = VS; v < 9; ++v) ldf[LD::QF + (v * N2 + p) * T + ci] = D[s][v];
+      ldf[LD::XB + p * T + ci] = db[s];
+    }
+  }
+  __syncthreads();
+  if (valid && fam == 1) {
+#pragma unroll
+    for (int s = 0; s < N; ++s) {
+      const int p = p0 + ps * s;
+#pragma unroll
+      for (int v = VS; v < 9; ++v) D[s][v] = ldf[LD::QF + (v * N2 + p) * T + ci] + D[s][v];
+      db[s] = ldf[LD::XB + p * T + ci];
+    }
+  }
+
+  // ---- solution points, in FP64: point (is, js) = (s, line) of a xi-line,
+  //      (line, s) of an eta-line; variables V0..V1-1 of it ----------------------
+  double rho_s[N], jz2_s[N];
+  auto points = [&](auto v0c, auto v1c) {
+    constexpr int V0 = decltype(v0c)::value, V1 = decltype(v1c)::value;
+#pragma unroll
+    for (int s = 0; s < N; ++s) {
+      const int p = p0 + ps * s;
+      const int is = fam ? line : s, js = fam ? s : line;
+      const double ji = mt[(G::JI + js + N * is) * T];
+      double R[9], q[9], xc[VS];
+#pragma unroll
+      for (int v = 0; v < 9; ++v)
+        if (needed(v, V0, V1)) q[v] = qn[v];
+#pragma unroll
+      for (int v = 0; v < V1 - V0; ++v) xc[v] = xn[v];
+#pragma unroll
+      for (int v = V0; v < V1; ++v) R[v] = -double(D[s][v]) * ji;   // compresid.f90:41,56
+      const double divB = double(db[s]) * ji;
       if (s + 1 < N) {
 #pragma unroll
-        for (int v = 0; v < 9; ++v) {
-          qn[v] = qg[(v * N2 + s + 1) * T];
-          if (NAUX == 1) xn[v] = aux0[(v * N2 + s + 1) * T];
+        for (int v = 0; v < 9; ++v)
+          if (needed(v, V0, V1)) qn[v] = qg[(v * N2 + p + ps) * T];
+        if (NAUX == 1) {
+#pragma unroll
+          for (int v = V0; v < V1; ++v) xn[v - V0] = aux0[(v * N2 + p + ps) * T];
         }
       }
       const double ri = 1.0 / q[0];
       const double ux = q[1] * ri, uy = q[2] * ri, uz = q[3] * ri;
       // Powell source and GLM damping, compresid.f90:58-74
-      R[1] -= divB * q[5];
-      R[2] -= divB * q[6];
-      R[3] -= divB * q[7];
-      R[4] -= divB * (ux * q[5] + uy * q[6] + uz * q[7]);
-      R[5] -= divB * ux;
-      R[6] -= divB * uy;
-      R[7] -= divB * uz;
-      R[8] -= ph.alpha * q[8];
-      if (AV) {
-        const double jz = double(dj[s] + ldf[LD::XJ + p * T + ci]) * ji;
+      if (V0 <= 1 && 1 < V1) R[1] -= divB * q[5];
+      if (V0 <= 2 && 2 < V1) R[2] -= divB * q[6];
+      if (V0 <= 3 && 3 < V1) R[3] -= divB * q[7];
+      if (V0 <= 4 && 4 < V1) R[4] -= divB * (ux * q[5] + uy * q[6] + uz * q[7]);
+      if (V0 <= 5 && 5 < V1) R[5] -= divB * ux;
+      if (V0 <= 6 && 6 < V1) R[6] -= divB * uy;
+      if (V0 <= 7 && 7 < V1) R[7] -= divB * uz;
+      if (V0 <= 8 && 8 < V1) R[8] -= ph.alpha * q[8];
+      if (AV && V0 == 0) {
+        const double jz = double(dj[s]) * ji;
         rho_s[s] = q[0];
         jz2_s[s] = jz * jz;
       }
       if (NORMS) {
 #pragma unroll
-        for (int v = 0; v < 9; ++v) nres = fma(R[v], R[v], nres);
-        ndiv = fma(divB, divB, ndiv);
+        for (int v = V0; v < V1; ++v) nres = fma(R[v], R[v], nres);
+        if (V0 == 0) ndiv = fma(divB, divB, ndiv);
       }
       const size_t off = (size_t)tile * G::QT + ci;
       double xs[9];
       if (NAUX > 1) {
 #pragma unroll
-        for (int v = 0; v < 9; ++v) xs[v] = 0.0;
+        for (int v = V0; v < V1; ++v) xs[v] = 0.0;
 #pragma unroll
         for (int i = 0; i < NAUX; ++i) {
 #pragma unroll
-          for (int v = 0; v < 9; ++v)
+          for (int v = V0; v < V1; ++v)
             xs[v] = fma(a.ca[i], a.aux[i][off + (size_t)(v * N2 + p) * T], xs[v]);
-          __builtin_amdgcn_sched_barrier(0);   // one register's 9 loads in flight at a time
+          __builtin_amdgcn_sched_barrier(0);   // one register's loads in flight at a time
         }
       }
 #pragma unroll
-      for (int v = 0; v < 9; ++v) {
+      for (int v = V0; v < V1; ++v) {
         const size_t o = off + (size_t)(v * N2 + p) * T;
         double acc = a.cin * q[v];
-        if (NAUX == 1) acc = fma(a.ca[0], xc[v], acc);
+        if (NAUX == 1) acc = fma(a.ca[0], xc[v - V0], acc);
         if (NAUX > 1) acc += xs[v];
         acc = fma(cr, R[v], acc);
         a.qout[o] = acc;
         if (RSTORE) a.rstore[o] = R[v];
       }
     }
+  };
+  if (valid) {
+    if (fam == 0) points(std::integral_constant<int, 0>{}, std::integral_constant<int, VS>{});
+    else points(std::integral_constant<int, VS>{}, std::integral_constant<int, 9>{});
   }
 
   // ---- block reductions -----------------------------------------------------------
