@@ -44,7 +44,6 @@ struct sd2d_ctx {
   double *fsr[4] = {nullptr, nullptr, nullptr, nullptr};  // FiveStageRk registers
   double *met = nullptr;
   int *ftab = nullptr;
-  int *torder = nullptr;
   int *c2s = nullptr, *s2c = nullptr;          // cell <-> slot maps of the tile shape
   double *avtab = nullptr;
   double *gmax = nullptr;                       // eigvmax_g, umax_g
@@ -312,7 +311,6 @@ static StageArgs base_args(sd2d_ctx *c) {
   StageArgs a{};
   a.met = c->met;
   a.ftab = c->ftab;
-  a.torder = c->torder;
   a.gmax = c->gmax;
   a.slots = c->slots;
   a.avtab = c->avtab;
@@ -367,7 +365,6 @@ static int ctx_upload(sd2d_ctx *c, const HostTables &tb, int nwall, int nmem,
     CK(hipMalloc(&c->metf, metf.size() * sizeof(float)));
   }
   CK(hipMalloc(&c->ftab, ft.size() * sizeof(int)));
-  CK(hipMalloc(&c->torder, tb.torder.size() * sizeof(int)));
   CK(hipMalloc(&c->c2s, tb.c2s.size() * sizeof(int)));
   CK(hipMalloc(&c->s2c, tb.s2c.size() * sizeof(int)));
   CK(hipMalloc(&c->avtab, av.size() * sizeof(double)));
@@ -391,8 +388,6 @@ static int ctx_upload(sd2d_ctx *c, const HostTables &tb, int nwall, int nmem,
     CK(hipMemcpyAsync(c->metf, metf.data(), metf.size() * sizeof(float), hipMemcpyHostToDevice,
                       c->stream));
   CK(hipMemcpyAsync(c->ftab, ft.data(), ft.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
-  CK(hipMemcpyAsync(c->torder, tb.torder.data(), tb.torder.size() * sizeof(int),
-                    hipMemcpyHostToDevice, c->stream));
   CK(hipMemcpyAsync(c->c2s, tb.c2s.data(), tb.c2s.size() * sizeof(int), hipMemcpyHostToDevice,
                     c->stream));
   CK(hipMemcpyAsync(c->s2c, tb.s2c.data(), tb.s2c.size() * sizeof(int), hipMemcpyHostToDevice,
@@ -483,7 +478,7 @@ void sd2d_destroy(sd2d_ctx *c) {
   adj_free(c->adj);
   for (double *b : c->q) if (b) (void)hipFree(b);
   for (double *b : c->fsr) if (b) (void)hipFree(b);
-  void *ptrs[] = {c->met, c->metf, c->ftab, c->torder, c->avtab, c->gmax, c->slots, c->sm_ind, c->normbuf,
+  void *ptrs[] = {c->met, c->metf, c->ftab, c->avtab, c->gmax, c->slots, c->sm_ind, c->normbuf,
                   c->normout, c->stagef, c->nablaf, c->gq, c->dei, c->visres, c->metw,
                   c->c2s, c->s2c, c->tinfo, c->dtm, c->phm};
   for (void *b : ptrs) if (b) (void)hipFree(b);

@@ -106,7 +106,6 @@ static int ens_build_group(sd2d_ens *e, int g, int shape) {
       gt.ft.push_back((slot_k % 2 == 0) ? tb.ft[k] + (int)(tiles * QT) : tb.ft[k]);
     }
     gt.met.insert(gt.met.end(), tb.met.begin(), tb.met.end());
-    for (int t : tb.torder) gt.torder.push_back(t + (int)tiles);
     for (int t = 0; t < tb.ntile; ++t) {
       const int nv = std::min(CPW, tb.ncell - t * CPW);
       tinfo.push_back((m.local << 6) | nv);

@@ -19,7 +19,6 @@ struct HostTables {
   std::vector<int> s2c;      // slot -> cell, -1 for the padding of the last tile
   Ops ops{};
   Phys ph{};
-  std::vector<int> torder;   // launch order of the tiles (see tile_order)
   std::vector<int> ft;       // [tile][k][{neighbour base, info}][ci]
   std::vector<double> met;   // [tile][metric slot][ci]
   std::vector<double> av;    // hlow[isl][is], then hhigh[is][isl]
@@ -291,10 +290,30 @@ inline int build_host_tables(const sd2d_mesh *m, const sd2d_params *p, HostTable
   tile_slots(m, CPW, p->tile_shape == SD2D_TILE_BLOCKS ? SD2D_TILE_BLOCKS : SD2D_TILE_STRIPS,
              t.c2s, t.s2c);
   std::vector<int> nb, info;
-  const int r = build_face_table(m, CPW, t.c2s, nb, info, err);
+  int r = build_face_table(m, CPW, t.c2s, nb, info, err);
   if (r) return r;
   const int nt = t.ntile;
-  t.torder = tile_order(nt, CPW, 9 * N2 * CPW, nb, info, t.c2s);
+  // the tiles are renumbered in launch order, so that a workgroup finds its
+  // tile from its position alone (no table read ahead of every other load);
+  // the tile holding the padding slots stays last, so tile t has
+  // min(CPW, ncell - t*CPW) valid cells
+  {
+    std::vector<int> ord = tile_order(nt, CPW, 9 * N2 * CPW, nb, info, t.c2s);
+    ord.erase(std::find(ord.begin(), ord.end(), nt - 1));
+    ord.push_back(nt - 1);
+    std::vector<int> pos(nt);
+    for (int i = 0; i < nt; ++i) pos[ord[i]] = i;
+    t.s2c.assign((size_t)nt * CPW, -1);
+    for (int cell = 0; cell < nc; ++cell) {
+      const int sl = t.c2s[cell];
+      t.c2s[cell] = pos[sl / CPW] * CPW + sl % CPW;
+      t.s2c[t.c2s[cell]] = cell;
+    }
+    nb.clear();
+    info.clear();
+    r = build_face_table(m, CPW, t.c2s, nb, info, err);
+    if (r) return r;
+  }
   t.ft.assign((size_t)nt * 8 * CPW, 0);
   for (int cell = 0; cell < nc; ++cell)
     for (int k = 0; k < 4; ++k) {

@@ -48,7 +48,6 @@ struct StageArgs {
   double *rstore;           // resid (tile layout) for FiveStageRk, or null
   const double *met;        // metrics (tile layout)
   const int *ftab;          // per-cell face table (tile layout)
-  const int *torder;        // launch order of the tiles
   const double *gmax;       // eigvmax_g, umax_g of the previous evaluation
   unsigned long long *slots;  // running maxima, NSLOT x 2 (bit patterns)
   double *sm_ind;           // SM_IND(2,NCELL) or null
@@ -307,7 +306,7 @@ __global__ __launch_bounds__(STAGE_THREADS, 2) void stage_kernel(StageArgs a, Op
   static_assert(N * T <= 64 && STAGE_THREADS == 128, "one wavefront per line family");
   extern __shared__ double lds[];
   const int tid = threadIdx.x;
-  const int tile = a.torder[tile_of_block(blockIdx.x, a.ntile, a.rev)];
+  const int tile = tile_of_block(blockIdx.x, a.ntile, a.rev);   // tiles are numbered in launch order
   // one family per wavefront (wave 0: xi-lines, wave 1: eta-lines), so the
   // branches that depend on the family (in-tile or gathered neighbour
   // traces, the points step) do not diverge inside a wave

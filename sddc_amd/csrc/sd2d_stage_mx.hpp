@@ -70,7 +70,7 @@ void stage_kernel_mx(StageArgs a, OpsF ops, const float *__restrict__ metf) {
   extern __shared__ double lds[];
   float *ldf = reinterpret_cast<float *>(lds);
   const int tid = threadIdx.x;
-  const int tile = a.torder[tile_of_block(blockIdx.x, a.ntile, a.rev)];
+  const int tile = tile_of_block(blockIdx.x, a.ntile, a.rev);   // tiles are numbered in launch order
   // one family per wavefront (wave 0: xi-lines, wave 1: eta-lines), so the
   // branches that depend on the family (in-tile or gathered neighbour
   // traces, the points step) do not diverge inside a wave
