@@ -473,6 +473,15 @@ __global__ __launch_bounds__(STAGE_THREADS, 2) void stage_kernel(StageArgs a, Op
   static_assert(VS <= LD::XV && 9 - VS <= LD::XV, "exchange area holds one round");
   double ax[N][VS];
   const double *aux0 = a.aux[0] + (size_t)tile * G::QT + ci;
+  // 1/J at the thread's points, with them
+  double jis[N];
+  if (valid) {
+#pragma unroll
+    for (int s = 0; s < N; ++s) {
+      const int is = fam ? line : s, js = fam ? s : line;
+      jis[s] = mt[(G::JI + js + N * is) * T];
+    }
+  }
   if (RK && NAUX == 1 && valid) {
     if (fam == 0) {
 #pragma unroll
@@ -569,8 +578,7 @@ __global__ __launch_bounds__(STAGE_THREADS, 2) void stage_kernel(StageArgs a, Op
 #pragma unroll
     for (int s = 0; s < N; ++s) {
       const int p = p0 + ps * s;
-      const int is = fam ? line : s, js = fam ? s : line;
-      const double ji = mt[(G::JI + js + N * is) * T];
+      const double ji = jis[s];
       double R[9], q[9];
 #pragma unroll
       for (int v = 0; v < 9; ++v) {
