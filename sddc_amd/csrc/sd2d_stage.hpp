@@ -216,6 +216,52 @@ __device__ __forceinline__ void face_gather_w(const StageArgs &a, const OP &ops,
   }
 }
 
+// The gathers of both end faces of a line from loaded face-table words, in
+// as few passes as the wave needs: pass 1 gathers, in every lane, the first
+// end face whose neighbour lies in another tile; pass 2 the N end of the
+// lanes that need both.  In a wave whose lanes need at most one gather each
+// (a line family whose neighbours are mostly in the tile: only the lanes of
+// the strip's end cells gather) pass 2 is skipped, which halves its gather
+// loads.  Each face's trace is formed exactly as by face_gather_w.
+template <int N, typename R, typename OP>
+__device__ __forceinline__ void face_gather_pair(const StageArgs &a, const OP &ops, int tile,
+                                                 int fam, int line, int nb0, int info0, int nbN,
+                                                 int infoN, R *qn0, R *qnN, int &lci0,
+                                                 int &lciN) {
+  constexpr int T = Geo<N>::CPW, N2 = Geo<N>::N2, QT = Geo<N>::QT;
+  const bool i0 = ((info0 >> 3) & 7) == FT_INTERIOR, iN = ((infoN >> 3) & 7) == FT_INTERIOR;
+  const bool g0 = i0 && nb0 / QT != tile, gN = iN && nbN / QT != tile;
+  lci0 = (i0 && !g0) ? nb0 - tile * QT : -1;
+  lciN = (iN && !gN) ? nbN - tile * QT : -1;
+  auto gather = [&](int rfp, int nbase, int info, R *Qn) {
+    int k, nfp;
+    end_face<N>(fam, line, rfp, k, nfp);
+    int p0, stride;
+    bool endN;
+    neighbour_line<N>(info & 3, nfp, p0, stride, endN);
+    R Lr[N];
+#pragma unroll
+    for (int s = 0; s < N; ++s) Lr[s] = endN ? ops.L[N][s] : ops.L[0][s];
+#pragma unroll
+    for (int v = 0; v < 9; ++v) {
+      R vals[N];
+#pragma unroll
+      for (int s = 0; s < N; ++s) vals[s] = R(a.qin[nbase + (v * N2 + p0 + stride * s) * T]);
+      Qn[v] = line_trace<N, R>(vals, Lr);
+    }
+  };
+  if (g0 || gN) {
+    R Q[9];
+    gather(g0 ? 0 : N, g0 ? nb0 : nbN, g0 ? info0 : infoN, Q);
+#pragma unroll
+    for (int v = 0; v < 9; ++v) {
+      if (g0) qn0[v] = Q[v];
+      else qnN[v] = Q[v];
+    }
+  }
+  if (g0 && gN) gather(N, nbN, infoN, qnN);
+}
+
 // Gather for an end face: the face-table info word and, for an interior or
 // periodic face, the neighbour's trace Qn at the shared face point (its
 // line through that point, interpolated with the same arithmetic as the
@@ -362,8 +408,8 @@ __global__ __launch_bounds__(STAGE_THREADS, 2) void stage_kernel(StageArgs a, Op
   double qn0[9], qnN[9];
   int lci0 = -1, lciN = -1;
   if (valid) {
-    face_gather_w<N, double>(a, ops, tile, fam, line, 0, nb0, info0, qn0, lci0);
-    face_gather_w<N, double>(a, ops, tile, fam, line, N, nbN, infoN, qnN, lciN);
+    face_gather_pair<N, double>(a, ops, tile, fam, line, nb0, info0, nbN, infoN, qn0, qnN, lci0,
+                                lciN);
   }
   // the metric vectors of the line's N+1 flux points, loaded while the
   // gathers are in flight

@@ -123,8 +123,8 @@ void stage_kernel_mx(StageArgs a, OpsF ops, const float *__restrict__ metf) {
   float qn0[9], qnN[9];
   int lci0 = -1, lciN = -1;
   if (valid) {
-    face_gather_w<N, float>(a, ops, tile, fam, line, 0, nb0, info0, qn0, lci0);
-    face_gather_w<N, float>(a, ops, tile, fam, line, N, nbN, infoN, qnN, lciN);
+    face_gather_pair<N, float>(a, ops, tile, fam, line, nb0, info0, nbN, infoN, qn0, qnN, lci0,
+                               lciN);
   }
   // the metric vectors of the line's N+1 flux points, loaded while the
   // gathers are in flight
