@@ -480,18 +480,10 @@ __global__ __launch_bounds__(STAGE_THREADS, 2) void stage_kernel(StageArgs a, Op
         dj[s] = fma(wjN, ops.M[N][s], wj0 * ops.M[0][s]);
       }
     }
-    // ---- interior flux points, one at a time (compflux.f90:53-104)
-#pragma unroll
-    for (int r = 1; r < N; ++r) {
-      __builtin_amdgcn_sched_barrier(0);
-      double tr[9];
-#pragma unroll
-      for (int v = 0; v < 9; ++v) {
-        double vals[N];
-#pragma unroll
-        for (int s = 0; s < N; ++s) vals[s] = ql[(v * N2 + ps * s) * T];
-        tr[v] = line_trace<N, double>(vals, ops.L[r]);
-      }
+    // ---- interior flux points, two at a time: one pass over the line's
+    //      values in LDS forms both traces, then each point's flux is folded
+    //      into the accumulators in order (compflux.f90:53-104)
+    auto interior_point = [&](int r, const double *tr) {
       const double sx = msx[r], sy = msy[r];
       double F[9];
       iflux_n<double>(tr, c_h, sx, sy, F, um);
@@ -504,6 +496,22 @@ __global__ __launch_bounds__(STAGE_THREADS, 2) void stage_kernel(StageArgs a, Op
         db[s] = fma(wb, ops.M[r][s], db[s]);
         dj[s] = fma(wj, ops.M[r][s], dj[s]);
       }
+    };
+#pragma unroll
+    for (int r = 1; r < N; r += 2) {
+      __builtin_amdgcn_sched_barrier(0);
+      const bool two = r + 1 < N;
+      double tr[9], tr2[9];
+#pragma unroll
+      for (int v = 0; v < 9; ++v) {
+        double vals[N];
+#pragma unroll
+        for (int s = 0; s < N; ++s) vals[s] = ql[(v * N2 + ps * s) * T];
+        tr[v] = line_trace<N, double>(vals, ops.L[r]);
+        if (two) tr2[v] = line_trace<N, double>(vals, ops.L[r + 1]);
+      }
+      interior_point(r, tr);
+      if (two) interior_point(r + 1, tr2);
     }
     __builtin_amdgcn_sched_barrier(0);
   }
