@@ -181,17 +181,9 @@ void stage_kernel_mx(StageArgs a, OpsF ops, const float *__restrict__ metf) {
         dj[s] = fmaf(wjN, ops.M[N][s], wj0 * ops.M[0][s]);
       }
     }
-#pragma unroll
-    for (int r = 1; r < N; ++r) {
-      __builtin_amdgcn_sched_barrier(0);
-      float tr[9];
-#pragma unroll
-      for (int v = 0; v < 9; ++v) {
-        float vals[N];
-#pragma unroll
-        for (int s = 0; s < N; ++s) vals[s] = ql[(v * N2 + ps * s) * T];
-        tr[v] = line_trace<N, float>(vals, ops.L[r]);
-      }
+    // interior flux points two at a time: one pass over the staged line
+    // values forms both traces, then each point is folded in, in order
+    auto interior_point = [&](int r, const float *tr) {
       const float sx = msx[r], sy = msy[r];
       float F[9];
       iflux_n<float>(tr, c_h, sx, sy, F, um);
@@ -204,6 +196,22 @@ void stage_kernel_mx(StageArgs a, OpsF ops, const float *__restrict__ metf) {
         db[s] = fmaf(wb, ops.M[r][s], db[s]);
         dj[s] = fmaf(wj, ops.M[r][s], dj[s]);
       }
+    };
+#pragma unroll
+    for (int r = 1; r < N; r += 2) {
+      __builtin_amdgcn_sched_barrier(0);
+      const bool two = r + 1 < N;
+      float tr[9], tr2[9];
+#pragma unroll
+      for (int v = 0; v < 9; ++v) {
+        float vals[N];
+#pragma unroll
+        for (int s = 0; s < N; ++s) vals[s] = ql[(v * N2 + ps * s) * T];
+        tr[v] = line_trace<N, float>(vals, ops.L[r]);
+        if (two) tr2[v] = line_trace<N, float>(vals, ops.L[r + 1]);
+      }
+      interior_point(r, tr);
+      if (two) interior_point(r + 1, tr2);
     }
     __builtin_amdgcn_sched_barrier(0);
   }
