@@ -57,6 +57,10 @@ struct sd2d_ctx {
   // free stream of every member (one entry for one run).
   int nmem = 1;
   long long nlaunch = 0;                // stage launches of sd2d_step so far
+  // the line family (0: xi, 1: eta) whose wave stages a tile into LDS: the
+  // one with fewer end faces whose neighbour lies in another tile (fewer
+  // neighbour gathers), from the face table
+  int sfam = 1;
   int *tinfo = nullptr;
   double *dtm = nullptr;
   Phys *phm = nullptr;
@@ -321,6 +325,7 @@ static StageArgs base_args(sd2d_ctx *c) {
   a.tinfo = c->tinfo;
   a.dtm = c->dtm;
   a.phm = c->phm;
+  a.sfam = c->sfam;
   return a;
 }
 
@@ -352,6 +357,20 @@ static int ctx_upload(sd2d_ctx *c, const HostTables &tb, int nwall, int nmem,
       return e_ == hipErrorOutOfMemory ? SD2D_ENOMEM : SD2D_EHIP;                  \
     }                                                                              \
   } while (0)
+  {
+    // end faces with the neighbour in another tile, per line family:
+    // xi-lines end at local faces 3 and 1, eta-lines at 0 and 2
+    const int QT = 9 * N2 * CPW;
+    long long cross[2] = {0, 0};
+    for (int t = 0; t < nt; ++t)
+      for (int k = 0; k < 4; ++k)
+        for (int ci = 0; ci < CPW; ++ci) {
+          const int nbase = ft[((size_t)t * 8 + 2 * k) * CPW + ci];
+          const int info = ft[((size_t)t * 8 + 2 * k + 1) * CPW + ci];
+          if (((info >> 3) & 7) == FT_INTERIOR && nbase / QT != t) ++cross[(k == 1 || k == 3) ? 0 : 1];
+        }
+    c->sfam = cross[0] < cross[1] ? 0 : 1;
+  }
   CK(hipGetDevice(&c->device));
   CK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   const size_t qn = (size_t)nt * 9 * N2 * CPW;

@@ -63,6 +63,7 @@ struct StageArgs {
   const int *tinfo;
   const double *dtm;
   int rev;                  // walk the launch order backwards (alternate stages)
+  int sfam;                 // line family whose wave stages the tile (fewer gathers)
   const Phys *phm;          // free stream and GLM alpha, one per member
   int ncell, ntile;
 };
@@ -383,24 +384,27 @@ __global__ __launch_bounds__(STAGE_THREADS, 2) void stage_kernel(StageArgs a, Op
     face_words<N>(a, tile, ci, fam, line, N, nbN, infoN);
   }
   // ---- the tile's state -> LDS in 16-byte loads (one contiguous block; all
-  //      loads are issued before the first LDS store) -------------------------
-  {
+  //      loads are issued before the first LDS store), by the wave of the
+  //      line family with fewer neighbours in other tiles, so that the two
+  //      waves issue about as many vector loads before the barrier ----------
+  if (fam == a.sfam) {
     constexpr int NV2 = 9 * N2 * T / 2;                  // double2 per tile
-    constexpr int NST = (NV2 + STAGE_THREADS - 1) / STAGE_THREADS;
+    constexpr int NSW = (NV2 + 63) / 64;                 // rounds of one wave
     static_assert((G::QT * 8) % 16 == 0, "tiles start on 16-byte boundaries");
     const double2 *src = reinterpret_cast<const double2 *>(a.qin + (size_t)tile * G::QT);
     double2 *dst = reinterpret_cast<double2 *>(lds + LD::Q0);
-    // rounds k < NST-1 are full for every thread (tid < STAGE_THREADS), so
-    // only the last one is guarded and the copy stays one basic block
-    double2 st[NST];
+    const int l = tid & 63;
+    // rounds k < NSW-1 are full for every lane, so only the last one is
+    // guarded and the copy stays one basic block
+    double2 st[NSW];
 #pragma unroll
-    for (int k = 0; k < NST; ++k) {
-      const int i = tid + k * STAGE_THREADS;
-      st[k] = src[(k < NST - 1 || i < NV2) ? i : 0];
+    for (int k = 0; k < NSW; ++k) {
+      const int i = l + k * 64;
+      st[k] = src[(k < NSW - 1 || i < NV2) ? i : 0];
     }
 #pragma unroll
-    for (int k = 0; k < NST - 1; ++k) dst[tid + k * STAGE_THREADS] = st[k];
-    if (tid + (NST - 1) * STAGE_THREADS < NV2) dst[tid + (NST - 1) * STAGE_THREADS] = st[NST - 1];
+    for (int k = 0; k < NSW - 1; ++k) dst[l + k * 64] = st[k];
+    if (l + (NSW - 1) * 64 < NV2) dst[l + (NSW - 1) * 64] = st[NSW - 1];
   }
   // ---- neighbour traces at both end faces: gathered from memory for a
   //      neighbour in another tile, formed from the staged tile after the

@@ -101,22 +101,24 @@ void stage_kernel_mx(StageArgs a, OpsF ops, const float *__restrict__ metf) {
     face_words<N>(a, tile, ci, fam, line, 0, nb0, info0);
     face_words<N>(a, tile, ci, fam, line, N, nbN, infoN);
   }
-  // ---- the tile's state -> LDS as FP32 --------------------------------------
-  {
+  // ---- the tile's state -> LDS as FP32, by the wave of the line family
+  //      with fewer neighbours in other tiles (as in stage_kernel) ------------
+  if (fam == a.sfam) {
     constexpr int NV2 = 9 * N2 * T / 2;
-    constexpr int NST = (NV2 + STAGE_THREADS - 1) / STAGE_THREADS;
+    constexpr int NSW = (NV2 + 63) / 64;
     const double2 *src = reinterpret_cast<const double2 *>(a.qin + (size_t)tile * G::QT);
     float2 *dst = reinterpret_cast<float2 *>(ldf + LD::QF);
-    double2 st[NST];
+    const int l = tid & 63;
+    double2 st[NSW];
 #pragma unroll
-    for (int k = 0; k < NST; ++k) {
-      const int i = tid + k * STAGE_THREADS;
-      st[k] = src[(k < NST - 1 || i < NV2) ? i : 0];
+    for (int k = 0; k < NSW; ++k) {
+      const int i = l + k * 64;
+      st[k] = src[(k < NSW - 1 || i < NV2) ? i : 0];
     }
 #pragma unroll
-    for (int k = 0; k < NST; ++k) {
-      const int i = tid + k * STAGE_THREADS;
-      if (k < NST - 1 || i < NV2) dst[i] = make_float2(float(st[k].x), float(st[k].y));
+    for (int k = 0; k < NSW; ++k) {
+      const int i = l + k * 64;
+      if (k < NSW - 1 || i < NV2) dst[i] = make_float2(float(st[k].x), float(st[k].y));
     }
   }
   // ---- neighbour traces at both end faces (memory or the staged tile) -------
