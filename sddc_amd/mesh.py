@@ -265,6 +265,109 @@ def read_restart(path: str, N: int):
     return Q, ctime, it
 
 
+def _ld_first(line: str) -> str:
+    """First item of a Fortran list-directed record: a quoted string, or the
+    text up to the first blank, comma or slash."""
+    t = line.strip()
+    if t[:1] in ("'", '"'):
+        q = t[0]
+        j = t.find(q, 1)
+        return t[1:j] if j > 0 else t[1:]
+    for k, ch in enumerate(t):
+        if ch in " \t,/":
+            return t[:k]
+    return t
+
+
+def _ld_float(tok: str) -> float:
+    """A list-directed real: D and Q exponent letters read as E."""
+    return float(tok.translate(str.maketrans("dDqQ", "eEeE")))
+
+
+@dataclass
+class QuadInput:
+    """QUAD.INP as readinput.f90:6-43 reads it."""
+    params: RunParams
+    namecel: str
+    namevrt: str
+    namevrt8: str
+    namebnd: str
+    metis_cell: str
+    namerestart: str
+    curve_wall: int
+    restart_ord: int
+
+
+def read_quad_inp(path: str) -> QuadInput:
+    """Read QUAD.INP record by record in the order of readinput.f90:8-43
+    (36 records, one item each; the comment records 1, 2, 9, 15, 19, 22,
+    24, 27 and 30 are skipped)."""
+    with open(path) as fh:
+        recs = fh.read().splitlines()
+    if len(recs) < 36:
+        raise ValueError(f"{path}: {len(recs)} records, QUAD.INP has 36")
+    it = iter(enumerate(recs, start=1))
+
+    def item(kind):
+        k, line = next(it)
+        tok = _ld_first(line)
+        if tok == "":
+            raise ValueError(f"{path}: record {k} is empty")
+        return tok if kind is str else (int(tok) if kind is int else _ld_float(tok))
+
+    def skip():
+        next(it)
+
+    skip(); skip()
+    names = [item(str) for _ in range(6)]
+    skip()
+    N, curve, vismode, mu, eta0 = item(int), item(int), item(int), item(float), item(float)
+    skip()
+    k_stage, dt, maxiter = item(int), item(float), item(int)
+    skip()
+    dxcycl, dycycl = item(float), item(float)
+    skip()
+    artiv = item(int)
+    skip()
+    nre, nwrite = item(int), item(int)
+    skip()
+    restart, restart_ord = item(int), item(int)
+    skip()
+    rinf, pinf, uinf, vinf, bxinf, byinf = (item(float) for _ in range(6))
+    p = RunParams(N=N, vismode=vismode, mu=mu, eta0=eta0, k_stage=k_stage, dt=dt,
+                  maxiter=maxiter, dxcycl=dxcycl, dycycl=dycycl, artiv=artiv, nre=nre,
+                  nwrite=nwrite, restart=restart, rinf=rinf, pinf=pinf, uinf=uinf,
+                  vinf=vinf, bxinf=bxinf, byinf=byinf)
+    return QuadInput(p, *names, curve_wall=curve, restart_ord=restart_ord)
+
+
+def _records(path: str):
+    with open(path) as fh:
+        return [ln.replace(",", " ").split() for ln in fh if ln.strip()]
+
+
+def read_mesh_files(cel: str, vrt: str, bnd: str, dxcycl: float = 0.0,
+                    dycycl: float = 0.0, name: str = "mesh") -> Mesh:
+    """The .cel, .vrt and .bnd files as READ_CELL_DATA, READ_VRT_DATA and
+    READ_BND_DATA read them (readGRIDinfo.f90:33,78,269): the i-th record
+    of each file is the i-th cell / vertex / boundary edge; the leading
+    number of a record is not used."""
+    rc = _records(cel)
+    cells = np.array([[int(r[1]), int(r[2]), int(r[3]), int(r[4])] for r in rc], dtype=np.int64)
+    rv = _records(vrt)
+    xv = np.array([_ld_float(r[1]) for r in rv])
+    yv = np.array([_ld_float(r[2]) for r in rv])
+    bd = []
+    for r in _records(bnd):
+        t = r[5].strip("'\"")[:4]
+        if t not in BC_TYPES:
+            raise ValueError(f"{bnd}: boundary type {r[5]!r} is not one of {BC_TYPES}")
+        bd.append((int(r[1]), int(r[2]), t))
+    if cells.size and (cells.min() < 1 or cells.max() > len(xv)):
+        raise ValueError(f"{cel}: vertex number outside 1..{len(xv)}")
+    return Mesh(cells, xv, yv, bd, dxcycl=dxcycl, dycycl=dycycl, name=name)
+
+
 # --------------------------------------------------------------------------
 # Topology (connectivity.f90, connect_bdry.f90, map_cyc_loc.f90)
 # --------------------------------------------------------------------------
