@@ -470,25 +470,48 @@ def build_topology(m: Mesh, dxcycl: float = None, dycycl: float = None
 
 def _cyclic_pairs(m: Mesh, topo: Topology, dxc: float, dyc: float):
     """Local periodic face pairs (connect_bdry.f90:281-308 classifies,
-    map_cyc_loc.f90:15-35 pairs).  The midpoint test is the reference's,
-    evaluated for all pairs at once."""
+    map_cyc_loc.f90:15-35 pairs) with the reference's midpoint test:
+    faces i, j pair when |xm_i - xm_j| is within TOLCYC of DXCYCL and
+    |ym_i - ym_j| < TOLCYC, or the same with x and y exchanged.  Candidates
+    come from a hash of the midpoints on a grid of spacing TOLCYC (a
+    partner lies in one of the 3 x 3 cells around the point shifted by a
+    period), so the cost is O(n) instead of the reference's all-pairs loop;
+    every candidate is then checked with the reference's test."""
     cyc = topo.lists["CYCL"]
     if len(cyc) == 0:
         return np.zeros((0, 2), dtype=np.int32)
     v = topo.if2v[cyc - 1] - 1
     xm = (m.xv[v[:, 0]] + m.xv[v[:, 1]]) / 2
     ym = (m.yv[v[:, 0]] + m.yv[v[:, 1]]) / 2
-    dx = np.abs(xm[:, None] - xm[None, :])
-    dy = np.abs(ym[:, None] - ym[None, :])
-    ex = np.abs(dx - dxc); ey = np.abs(dy - dyc)
-    match = ((ex < TOLCYC) & (dy < TOLCYC)) | ((ey < TOLCYC) & (dx < TOLCYC))
-    local = match.any(axis=1)
-    if not local.all():
+    n = len(cyc)
+    h = TOLCYC
+    grid = {}
+    for i in range(n):
+        grid.setdefault((int(math.floor(xm[i] / h)), int(math.floor(ym[i] / h))), []).append(i)
+    pairs = set()
+    matched = np.zeros(n, dtype=bool)
+    for i in range(n):
+        for sx, sy in ((dxc, 0.0), (-dxc, 0.0), (0.0, dyc), (0.0, -dyc)):
+            if sx == 0.0 and sy == 0.0:
+                continue
+            cx = int(math.floor((xm[i] + sx) / h))
+            cy = int(math.floor((ym[i] + sy) / h))
+            for gx in (cx - 1, cx, cx + 1):
+                for gy in (cy - 1, cy, cy + 1):
+                    for j in grid.get((gx, gy), ()):
+                        if j == i:
+                            continue
+                        dx = abs(xm[i] - xm[j])
+                        dy = abs(ym[i] - ym[j])
+                        if (abs(dx - dxc) < TOLCYC and dy < TOLCYC) or \
+                           (abs(dy - dyc) < TOLCYC and dx < TOLCYC):
+                            matched[i] = True
+                            pairs.add((min(i, j), max(i, j)))
+    if not matched.all():
         raise ValueError("periodic faces without a local partner")
-    iu = np.triu(match, k=1)
-    i1, i2 = np.nonzero(iu)
-    if len(i1) != len(cyc) // 2:
+    if len(pairs) != n // 2:
         raise ValueError("error in matching CYCLOC faces")
+    i1, i2 = np.array(sorted(pairs)).T
     return np.stack([cyc[i1], cyc[i2]], axis=1).astype(np.int32)
 
 

@@ -108,3 +108,29 @@ def test_run_refuses_curved_walls(tmp_path):
     (tmp_path / "QUAD.INP").write_text("\n".join(txt) + "\n")
     with pytest.raises(NotImplementedError):
         R.run_dir(str(tmp_path))
+
+
+def _pairs_all(m, topo, dxc, dyc):
+    """All-pairs form of the reference's midpoint test (map_cyc_loc.f90:15-35)."""
+    cyc = topo.lists["CYCL"]
+    v = topo.if2v[cyc - 1] - 1
+    xm = (m.xv[v[:, 0]] + m.xv[v[:, 1]]) / 2
+    ym = (m.yv[v[:, 0]] + m.yv[v[:, 1]]) / 2
+    out = []
+    for i in range(len(cyc)):
+        for j in range(i + 1, len(cyc)):
+            dx, dy = abs(xm[i] - xm[j]), abs(ym[i] - ym[j])
+            if (abs(dx - dxc) < M.TOLCYC and dy < M.TOLCYC) or \
+               (abs(dy - dyc) < M.TOLCYC and dx < M.TOLCYC):
+                out.append((cyc[i], cyc[j]))
+    return np.array(out, dtype=np.int32)
+
+
+@pytest.mark.parametrize("nx,ny", [(4, 4), (5, 3), (16, 7), (1, 6)])
+def test_cyclic_pairs_match_all_pairs_test(nx, ny):
+    """The hashed pairing returns exactly the pairs, in the same order, as
+    the all-pairs test, also on meshes with one cell across."""
+    m = M.periodic_square(nx, ny)
+    topo = M.build_topology(m, m.dxcycl, m.dycycl)
+    ref = _pairs_all(m, topo, m.dxcycl, m.dycycl)
+    assert np.array_equal(topo.cyc2fpair, ref)
