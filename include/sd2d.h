@@ -181,6 +181,20 @@ int sd2d_compresid_q(sd2d_ctx *ctx, const double *q,
  */
 int sd2d_step(sd2d_ctx *ctx, int32_t nsteps, int32_t norm_every);
 
+/*
+ * One stage (1..5) of the current step of the configured scheme, for hosts
+ * that drive the stages themselves (the calls of compresid inside RK_S5P3,
+ * iterations.f90:183-209, and FiveStageRk, iterations.f90:107-144).
+ * Stages run in order: `stage` must be the next one of the step
+ * (SD2D_EINVAL otherwise); stage 1 starts a step (iter + 1) and stage 5
+ * completes it (the new state becomes the one sd2d_get_state returns, and
+ * ctime advances by dt).  Five calls give bitwise the result of one
+ * sd2d_step(ctx, 1, 0).  No norms are formed.  While a step is in
+ * progress, sd2d_step and sd2d_fwd return SD2D_EINVAL; sd2d_set_state with
+ * a state abandons the step.
+ */
+int sd2d_rk_stage(sd2d_ctx *ctx, int32_t stage);
+
 /* Diagnostics of the latest stage / step. */
 int sd2d_get_diag(sd2d_ctx *ctx, sd2d_diag *diag);
 

@@ -57,6 +57,7 @@ struct sd2d_ctx {
   // free stream of every member (one entry for one run).
   int nmem = 1;
   long long nlaunch = 0;                // stage launches of sd2d_step so far
+  int next_stage = 1;                   // sd2d_rk_stage: the next stage of the step
   // the line family (0: xi, 1: eta) whose wave stages a tile into LDS: the
   // one with fewer end faces whose neighbour lies in another tile (fewer
   // neighbour gathers), from the face table
@@ -535,6 +536,7 @@ int sd2d_set_state(sd2d_ctx *c, const double *q, const sd2d_maxima *mx) {
     HIPCHK(c, hipMemcpyAsync(c->stagef, q, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
     int r = f2t(c, c->stagef, c->q[c->cur]);
     if (r) return r;
+    c->next_stage = 1;   // a step in progress is abandoned
   }
   if (mx) {
     hipLaunchKernelGGL(k_set_max, dim3(1), dim3(NSLOT), 0, c->stream, c->slots, c->gmax,
@@ -618,10 +620,16 @@ int sd2d_compresid_q(sd2d_ctx *c, const double *q, double *resid_out, double *na
   return sd2d_compresid(c, resid_out, nablaqs_out);
 }
 
-static int step_rk(sd2d_ctx *c, bool norms) {
+// One RK step, or (only > 0) only stage `only` of it: the sequence is walked
+// as a whole and the other stages' launches are skipped; the buffers
+// persist between calls, so stages run one call at a time give the same
+// result.  The rotation and ctime advance with the last stage.
+static int step_rk(sd2d_ctx *c, bool norms, int only = 0) {
   RkBuffers rb{{c->q[0], c->q[1], c->q[2]}, {c->fsr[0], c->fsr[1], c->fsr[2]}, c->cur};
   const int av = (c->p.artiv == 1 && c->p.vismode == 1) ? M_AV : 0;
+  int idx = 0;
   auto launch = [&](StageArgs &a, int mode) -> int {
+    if (only > 0 && ++idx != only) return 0;
     if (mode & M_NORMS) a.normbuf = c->normbuf;
     a.rev = c->nlaunch++ & 1;   // alternate the tile order stage by stage
     if (c->vis) {
@@ -635,6 +643,7 @@ static int step_rk(sd2d_ctx *c, bool norms) {
   const int r = (c->p.k_stage == 5) ? fsrk_sequence(rb, c->p.dt, norms, av, base_args(c), launch)
                                     : s5p3_sequence(rb, c->p.dt, norms, av, base_args(c), launch);
   if (r) return r;
+  if (only > 0 && only < 5) return SD2D_OK;
   c->cur = rb.cur;
   // ctime bookkeeping of RK_S5P3 / FiveStageRk: only the final value
   // (ctime0 + dt) survives a step
@@ -642,9 +651,26 @@ static int step_rk(sd2d_ctx *c, bool norms) {
   return SD2D_OK;
 }
 
+int sd2d_rk_stage(sd2d_ctx *c, int32_t stage) {
+  if (!c) return fail(nullptr, SD2D_EINVAL, "ctx is NULL");
+  if (stage != c->next_stage)
+    return fail(c, SD2D_EINVAL, "stage " + std::to_string(stage) + " out of order: the next stage is " +
+                                    std::to_string(c->next_stage));
+  if (stage == 1) c->iter += 1;
+  const int r = step_rk(c, false, stage);
+  if (r) return r;
+  c->next_stage = stage == 5 ? 1 : stage + 1;
+  if (stage == 5) c->have_av = c->p.artiv == 1 && c->p.vismode == 1;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (c->prof) return collect_profile(c);
+  return SD2D_OK;
+}
+
 int sd2d_step(sd2d_ctx *c, int32_t nsteps, int32_t norm_every) {
   if (!c) return fail(nullptr, SD2D_EINVAL, "ctx is NULL");
   if (nsteps < 0) return fail(c, SD2D_EINVAL, "nsteps < 0");
+  if (c->next_stage != 1)
+    return fail(c, SD2D_EINVAL, "a step is in progress (sd2d_rk_stage): finish it first");
   for (int s = 0; s < nsteps; ++s) {
     c->iter += 1;
     const bool norms = norm_every > 0 && (c->iter % norm_every) == 1;

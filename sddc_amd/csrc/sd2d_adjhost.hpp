@@ -329,6 +329,8 @@ extern "C" {
 int sd2d_fwd(sd2d_ctx *c, int32_t nsteps, int32_t ckpt, double *objective) {
   if (!c) return fail(nullptr, SD2D_EINVAL, "ctx is NULL");
   if (nsteps <= 0 || ckpt <= 0) return fail(c, SD2D_EINVAL, "nsteps and ckpt must be positive");
+  if (c->next_stage != 1)
+    return fail(c, SD2D_EINVAL, "a step is in progress (sd2d_rk_stage): finish it first");
   if (c->p.precision != SD2D_FP64 || c->p.k_stage != 3 || c->vis || c->tinfo)
     return fail(c, SD2D_EUNSUPPORTED,
                 "the adjoint covers the FP64 inviscid path with RK_S5P3 (k_stage 3)");

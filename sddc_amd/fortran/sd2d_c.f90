@@ -58,7 +58,7 @@ module sd2d_c
 
   public :: sd2d_create, sd2d_destroy, sd2d_last_error, sd2d_set_state, &
        sd2d_get_state, sd2d_set_time, sd2d_compresid, sd2d_compresid_q, &
-       sd2d_step, sd2d_get_diag, sd2d_get_av, sd2d_write_restart, &
+       sd2d_step, sd2d_rk_stage, sd2d_get_diag, sd2d_get_av, sd2d_write_restart, &
        sd2d_read_restart, sd2d_set_profiling, sd2d_get_profile, sd2d_error_message, &
        sd2d_ens_create, sd2d_ens_destroy, sd2d_ens_last_error, sd2d_ens_set_state, &
        sd2d_ens_get_state, sd2d_ens_set_time, sd2d_ens_get_time, sd2d_ens_step, &
@@ -124,6 +124,12 @@ module sd2d_c
        type(c_ptr), value :: ctx
        integer(c_int32_t), value :: nsteps, norm_every
      end function sd2d_step
+
+     integer(c_int) function sd2d_rk_stage(ctx, stage) bind(C, name="sd2d_rk_stage")
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: stage
+     end function sd2d_rk_stage
 
      integer(c_int) function sd2d_get_diag(ctx, diag) bind(C, name="sd2d_get_diag")
        import :: c_int, c_ptr, sd2d_diag_t
