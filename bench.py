@@ -257,8 +257,9 @@ def main() -> int:
     ap.add_argument("--t-end", type=float, default=None,
                     help="instead of the throughput run: FP64 and mixed runs to this "
                          "time, reporting the mixed form's L2 difference")
-    ap.add_argument("--cpu-steps", type=int, default=1,
-                    help="steps of the CPU baseline sample (0 = skip)")
+    ap.add_argument("--cpu-steps", type=int, default=3,
+                    help="steps of the CPU baseline sample (0 = skip); 3 C2 steps are "
+                         "about 10 s of the oracle on 16 host threads")
     ap.add_argument("--tile-shape", default="strips", choices=["strips", "blocks"],
                     help="cells of one workgroup: mesh-order strips (default) or compact blocks")
     ap.add_argument("--no-mixed", action="store_true",
