@@ -263,7 +263,7 @@ static hipError_t launch_vis_n(sd2d_ctx *c, const double *qin) {
   va.qin = qin; va.ftab = c->ftab; va.met = c->met; va.metw = c->metw;
   va.gq = c->gq; va.dei = c->dei; va.visres = c->visres;
   va.mu = c->p.mu; va.eta0 = c->p.eta0;
-  va.ncell = c->ncell; va.ntile = c->ntile; va.walls = c->dei != nullptr;
+  va.ncell = c->ncell; va.ntile = c->ntile; va.walls = c->dei != nullptr; va.s2c = c->s2c;
   constexpr size_t lds = vis_lds_bytes<N>();
   static_assert(lds <= 65536, "viscous kernel LDS above 64 KiB");
   hipLaunchKernelGGL(vis_grad_kernel<N>, dim3(c->ntile), dim3(STAGE_THREADS), lds, c->stream,
@@ -338,7 +338,8 @@ static int finalize_stage(sd2d_ctx *c) {
 
 // Device allocations and uploads of a context from its host tables.  An
 // ensemble group passes its members' per-tile info, time steps and free
-// streams (nmem > 1 entries of the maxima); one run passes null.  On error
+// streams (nmem > 1 entries of the maxima); one run passes its per-tile
+// valid-cell counts as tinfo (member 0) and no time steps.  On error
 // the context keeps what was allocated (sd2d_destroy frees it).
 static int ctx_upload(sd2d_ctx *c, const HostTables &tb, int nwall, int nmem,
                       const std::vector<int> *tinfo, const std::vector<double> *dtm,
@@ -417,9 +418,11 @@ static int ctx_upload(sd2d_ctx *c, const HostTables &tb, int nwall, int nmem,
   CK(hipMemsetAsync(c->sm_ind, 0, 2 * (size_t)nc * sizeof(double), c->stream));
   if (tinfo) {
     CK(hipMalloc(&c->tinfo, tinfo->size() * sizeof(int)));
-    CK(hipMalloc(&c->dtm, dtm->size() * sizeof(double)));
     CK(hipMemcpyAsync(c->tinfo, tinfo->data(), tinfo->size() * sizeof(int), hipMemcpyHostToDevice,
                       c->stream));
+  }
+  if (dtm) {
+    CK(hipMalloc(&c->dtm, dtm->size() * sizeof(double)));
     CK(hipMemcpyAsync(c->dtm, dtm->data(), dtm->size() * sizeof(double), hipMemcpyHostToDevice,
                       c->stream));
   }
@@ -485,7 +488,9 @@ int sd2d_create(const sd2d_mesh *m, const sd2d_params *p, sd2d_ctx **out) {
   c->vis = vis;
   c->xs.assign(m->xs, m->xs + N);
   {
-    const int r = ctx_upload(c, tb, m->nwall, 1, nullptr, nullptr, nullptr);
+    // one run: member 0 and the valid cells of every tile
+    const std::vector<int> tinfo(tb.tvalid.begin(), tb.tvalid.end());
+    const int r = ctx_upload(c, tb, m->nwall, 1, &tinfo, nullptr, nullptr);
     if (r) { g_err = c->err; sd2d_destroy(c); return r; }
   }
   *out = c;

@@ -90,8 +90,8 @@ __global__ __launch_bounds__(STAGE_THREADS) void adj_point_kernel(AdjArgs a) {
   for (int i = threadIdx.x; i < N2 * T; i += STAGE_THREADS) {
     const int p = i / T, ci = i % T;
     const int slot = tile * T + ci;
-    if (slot >= a.ncell) continue;
     const int cell = a.s2c[slot];
+    if (cell < 0) continue;   // padding slot
     const int is = p % N, js = p / N;
     const double ji = mt[(G::JI + js + N * is) * T + ci];
     const size_t off = (size_t)tile * G::QT + ci;
@@ -285,7 +285,7 @@ __global__ __launch_bounds__(STAGE_THREADS) void adj_line_kernel(AdjArgs a, Ops 
   const int tid = threadIdx.x, tile = blockIdx.x;
   const int ci = tid % T, fl = tid / T;
   const int fam = fl >= N ? 1 : 0, line = fl - fam * N;
-  const bool valid = tid < NT && tile * T + ci < a.ncell;
+  const bool valid = tid < NT && a.s2c[tile * T + ci] >= 0;
   const Ops &ops = ops_to_lds<LD::OPS>(lds, ops_arg);
   const double ge = a.gm[0], gu = a.gm[1];
   const double c_h = sqrt(ge * (ge - gu));   // as the forward forms it (compflux.f90:6)
@@ -353,7 +353,7 @@ __global__ void drag_kernel(const double *q, const int *ftab, const double *met,
   if (slot >= ntile * T) return;
   const int tile = slot / T, ci = slot % T;
   double d = 0.0;
-  if (slot < ncell) {
+  {   // (a padding slot has no wall face: d stays 0)
     const double *qc = q + (size_t)tile * G::QT + ci;
     const double *mt = met + (size_t)tile * G::NM * T + ci;
     for (int k = 0; k < 4; ++k) {
@@ -478,7 +478,7 @@ __global__ __launch_bounds__(STAGE_THREADS) void adj_argmax_kernel(AdjArgs a, Op
   const int tid = threadIdx.x, tile = blockIdx.x;
   const int ci = tid % T, fl = tid / T;
   const int fam = fl >= N ? 1 : 0, line = fl - fam * N;
-  if (!(tid < NT && tile * T + ci < a.ncell)) return;
+  if (!(tid < NT && a.s2c[tile * T + ci] >= 0)) return;
   const int p0 = fam ? line : N * line, ps = fam ? N : 1;
   double be = 0.0, bu = 0.0;
   for (int r = 0; r < NF; ++r) {

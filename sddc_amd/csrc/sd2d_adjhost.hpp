@@ -331,7 +331,7 @@ int sd2d_fwd(sd2d_ctx *c, int32_t nsteps, int32_t ckpt, double *objective) {
   if (nsteps <= 0 || ckpt <= 0) return fail(c, SD2D_EINVAL, "nsteps and ckpt must be positive");
   if (c->next_stage != 1)
     return fail(c, SD2D_EINVAL, "a step is in progress (sd2d_rk_stage): finish it first");
-  if (c->p.precision != SD2D_FP64 || c->p.k_stage != 3 || c->vis || c->tinfo)
+  if (c->p.precision != SD2D_FP64 || c->p.k_stage != 3 || c->vis || c->nmem > 1)
     return fail(c, SD2D_EUNSUPPORTED,
                 "the adjoint covers the FP64 inviscid path with RK_S5P3 (k_stage 3)");
   if (!c->adj) c->adj = new (std::nothrow) AdjState();
@@ -435,7 +435,7 @@ int sd2d_vjp(sd2d_ctx *c, const double *qbar_final, double jbar, double *qbar_in
 
 int sd2d_compresid_vjp(sd2d_ctx *c, const double *rbar, double *qbar, double *chbar, double *pbar) {
   if (!c || !rbar) return fail(c, SD2D_EINVAL, "NULL argument");
-  if (c->p.precision != SD2D_FP64 || c->vis || c->tinfo)
+  if (c->p.precision != SD2D_FP64 || c->vis || c->nmem > 1)
     return fail(c, SD2D_EUNSUPPORTED, "the adjoint covers the FP64 inviscid path");
   if (!c->adj) c->adj = new (std::nothrow) AdjState();
   AdjState *A = c->adj;

@@ -107,7 +107,7 @@ static int ens_build_group(sd2d_ens *e, int g, int shape) {
     }
     gt.met.insert(gt.met.end(), tb.met.begin(), tb.met.end());
     for (int t = 0; t < tb.ntile; ++t) {
-      const int nv = std::min(CPW, tb.ncell - t * CPW);
+      const int nv = tb.tvalid[t];
       tinfo.push_back((m.local << 6) | nv);
     }
     for (int sl = 0; sl < tb.ntile * CPW; ++sl)

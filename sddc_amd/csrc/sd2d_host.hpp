@@ -3,6 +3,8 @@
 // built once from the arrays module setup2d holds after INIT_SETUP.
 #pragma once
 #include <algorithm>
+#include <cstdlib>
+#include <map>
 #include <cstdint>
 #include <string>
 #include <utility>
@@ -20,6 +22,7 @@ struct HostTables {
   Ops ops{};
   Phys ph{};
   std::vector<int> ft;       // [tile][k][{neighbour base, info}][ci]
+  std::vector<int> tvalid;   // valid cells of each tile (its first slots)
   std::vector<double> met;   // [tile][metric slot][ci]
   std::vector<double> av;    // hlow[isl][is], then hhigh[is][isl]
   std::vector<double> metw;  // viscous path: other-family metrics at line ends
@@ -197,28 +200,62 @@ inline std::vector<int> bisection_order(int n, int grain, const std::vector<int>
   return ord;
 }
 
-// Cell -> slot map of a tile shape.  Strips: slot = cell.  Blocks: the
-// cells in the order of a recursive bisection of the cell adjacency graph
-// (interior faces and periodic pairs) with one tile as the grain, so every
-// tile is a compact patch of cells; the padding stays at the end.
+// Row length of a mesh numbered row by row: R when every interior face and
+// periodic pair joins cells c and c+1 of one row, c and c+R (next row), the
+// first and last cells of a row (c and c+R-1, a periodic row) or a cell of
+// the first and of the last row (c and c+nc-R, periodic columns); 0
+// otherwise.  R is the most frequent distance above 1.
+inline int mesh_row_length(int nc, const std::vector<std::pair<int, int>> &pairs) {
+  std::map<int, long long> hist;
+  for (const auto &pr : pairs) {
+    const int d = std::abs(pr.first - pr.second);
+    if (d > 1) ++hist[d];
+  }
+  int R = 0;
+  long long best = 0;
+  for (const auto &h : hist)
+    if (h.second > best) { best = h.second; R = h.first; }
+  if (R < 2 || nc % R != 0) return 0;
+  for (const auto &pr : pairs) {
+    const int lo = std::min(pr.first, pr.second), d = std::abs(pr.first - pr.second);
+    const bool ok = (d == 1 && lo % R != R - 1) || d == R || (d == R - 1 && lo % R == 0) ||
+                    (d == nc - R && lo < R);
+    if (!ok) return 0;
+  }
+  return R;
+}
+
+// Cell -> slot map of a tile shape.  Strips: slot = cell, except that a
+// mesh numbered row by row with rows of R >= CPW cells, R not a multiple of
+// CPW, gets every row padded to a multiple of CPW (slot = row*Rpad + column)
+// when that adds at most 1/16 of the row
+// so that the tiles of one row line up with those of the next and a tile's
+// neighbours across the rows lie in one tile.  Blocks: the cells in the
+// order of a recursive bisection of the cell adjacency graph (interior
+// faces and periodic pairs) with one tile as the grain, so every tile is a
+// compact patch of cells.  Either way the padding slots of a tile come
+// after its cells; s2c is -1 there.
 inline void tile_slots(const sd2d_mesh *m, int CPW, int shape, std::vector<int> &c2s,
                        std::vector<int> &s2c) {
-  const int nc = m->ncell, nt = (nc + CPW - 1) / CPW;
+  const int nc = m->ncell;
+  std::vector<std::pair<int, int>> pairs;   // 0-based cells joined by a face
+  auto link = [&](int f1, int k1, int f2, int k2) {
+    if (f1 < 1 || f1 > m->nface || f2 < 1 || f2 > m->nface) return;
+    const int a = m->if2c[(f1 - 1) + (size_t)m->nface * k1] - 1;
+    const int b = m->if2c[(f2 - 1) + (size_t)m->nface * k2] - 1;
+    if (a >= 0 && a < nc && b >= 0 && b < nc && a != b) pairs.push_back({a, b});
+  };
+  for (int i = 0; i < m->ninter; ++i) link(m->ibfinter[i], 0, m->ibfinter[i], 1);
+  for (int i = 0; i < m->npair; ++i) link(m->cyc2fpair[i], 0, m->cyc2fpair[i + m->npair], 0);
   std::vector<int> order(nc);
   for (int c = 0; c < nc; ++c) order[c] = c;
+  c2s.assign(nc, 0);
   if (shape == SD2D_TILE_BLOCKS) {
     std::vector<std::vector<int>> adj(nc);
-    auto link = [&](int f1, int k1, int f2, int k2) {
-      if (f1 < 1 || f1 > m->nface || f2 < 1 || f2 > m->nface) return;
-      const int a = m->if2c[(f1 - 1) + (size_t)m->nface * k1] - 1;
-      const int b = m->if2c[(f2 - 1) + (size_t)m->nface * k2] - 1;
-      if (a >= 0 && a < nc && b >= 0 && b < nc && a != b) {
-        adj[a].push_back(b);
-        adj[b].push_back(a);
-      }
-    };
-    for (int i = 0; i < m->ninter; ++i) link(m->ibfinter[i], 0, m->ibfinter[i], 1);
-    for (int i = 0; i < m->npair; ++i) link(m->cyc2fpair[i], 0, m->cyc2fpair[i + m->npair], 0);
+    for (const auto &pr : pairs) {
+      adj[pr.first].push_back(pr.second);
+      adj[pr.second].push_back(pr.first);
+    }
     std::vector<int> xadj(nc + 1, 0), ladj;
     for (int c = 0; c < nc; ++c) {
       std::sort(adj[c].begin(), adj[c].end());
@@ -227,8 +264,19 @@ inline void tile_slots(const sd2d_mesh *m, int CPW, int shape, std::vector<int> 
       ladj.insert(ladj.end(), adj[c].begin(), adj[c].end());
     }
     order = bisection_order(nc, CPW, xadj, ladj);
+  } else {
+    const int R = mesh_row_length(nc, pairs);
+    const int Rpad = R > 0 ? (R + CPW - 1) / CPW * CPW : 0;
+    if (R >= CPW && Rpad != R && 16 * (Rpad - R) <= R) {   // padding at most 1/16
+      s2c.assign((size_t)(nc / R) * Rpad, -1);
+      for (int c = 0; c < nc; ++c) {
+        c2s[c] = (c / R) * Rpad + c % R;
+        s2c[c2s[c]] = c;
+      }
+      return;
+    }
   }
-  c2s.assign(nc, 0);
+  const int nt = (nc + CPW - 1) / CPW;
   s2c.assign((size_t)nt * CPW, -1);
   for (int sl = 0; sl < nc; ++sl) {
     c2s[order[sl]] = sl;
@@ -289,18 +337,17 @@ inline int build_host_tables(const sd2d_mesh *m, const sd2d_params *p, HostTable
   }
   tile_slots(m, CPW, p->tile_shape == SD2D_TILE_BLOCKS ? SD2D_TILE_BLOCKS : SD2D_TILE_STRIPS,
              t.c2s, t.s2c);
+  t.ntile = (int)(t.s2c.size() / CPW);
   std::vector<int> nb, info;
   int r = build_face_table(m, CPW, t.c2s, nb, info, err);
   if (r) return r;
   const int nt = t.ntile;
   // the tiles are renumbered in launch order, so that a workgroup finds its
   // tile from its position alone (no table read ahead of every other load);
-  // the tile holding the padding slots stays last, so tile t has
-  // min(CPW, ncell - t*CPW) valid cells
+  // a tile keeps the positions of its cells, so its valid cells stay a
+  // prefix of it (tvalid)
   {
     std::vector<int> ord = tile_order(nt, CPW, 9 * N2 * CPW, nb, info, t.c2s);
-    ord.erase(std::find(ord.begin(), ord.end(), nt - 1));
-    ord.push_back(nt - 1);
     std::vector<int> pos(nt);
     for (int i = 0; i < nt; ++i) pos[ord[i]] = i;
     t.s2c.assign((size_t)nt * CPW, -1);
@@ -321,9 +368,23 @@ inline int build_host_tables(const sd2d_mesh *m, const sd2d_params *p, HostTable
       put_tile<int>(t.ft, 8, CPW, t.c2s[cell], 2 * k + 1, info[4 * (size_t)cell + k]);
     }
   // padded slots: FREE boundary so that nothing is dereferenced for them
-  for (int sl = nc; sl < nt * CPW; ++sl)
-    for (int k = 0; k < 4; ++k)
-      put_tile<int>(t.ft, 8, CPW, sl, 2 * k + 1, finfo_pack(0, 1, FT_FREE, 0));
+  for (int sl = 0; sl < nt * CPW; ++sl)
+    if (t.s2c[sl] < 0)
+      for (int k = 0; k < 4; ++k)
+        put_tile<int>(t.ft, 8, CPW, sl, 2 * k + 1, finfo_pack(0, 1, FT_FREE, 0));
+  // valid cells per tile; the kernels take a tile's cells as the first
+  // tvalid[t] slots of it
+  t.tvalid.assign(nt, 0);
+  for (int tt = 0; tt < nt; ++tt) {
+    int nv = 0;
+    while (nv < CPW && t.s2c[(size_t)tt * CPW + nv] >= 0) ++nv;
+    for (int ci = nv; ci < CPW; ++ci)
+      if (t.s2c[(size_t)tt * CPW + ci] >= 0) {
+        err = "internal error: the cells of a tile are not a prefix of it";
+        return SD2D_EINVAL;
+      }
+    t.tvalid[tt] = nv;
+  }
 
   // metrics: S1(1,1:2,rfp,js), S2(2,1:2,is,rfp), 1/Jac(is,js)
   const int NM = 4 * N * (N + 1) + N2;

@@ -45,6 +45,7 @@ struct VisArgs {
   double *gq;              // gradient [tile][(v*2+d)*N2 + p][ci]
   double *dei;             // d(e)/dxi, d(e)/deta [tile][k*N2 + p][ci] (walls)
   double *visres;          // dFv + dGv [tile][v*N2 + p][ci]
+  const int *s2c;          // slot -> mesh cell, -1 for a padding slot
   double mu, eta0;
   int ncell, ntile, walls;
 };
@@ -169,8 +170,8 @@ __global__ __launch_bounds__(STAGE_THREADS) void vis_grad_kernel(VisArgs va, Ops
   const int tid = threadIdx.x, tile = blockIdx.x;
   const int ci = tid % T, fl = tid / T;
   const int fam = fl >= N ? 1 : 0, line = fl - fam * N;
-  const int cell = tile * T + ci;
-  const bool valid = tid < NT && cell < va.ncell;
+  const int cell = tile * T + ci;   // the slot
+  const bool valid = tid < NT && va.s2c[cell] >= 0;
   if (valid) {
 #pragma unroll 1
     for (int r = 0; r < NF; ++r) {
@@ -403,8 +404,8 @@ __global__ __launch_bounds__(STAGE_THREADS) void vis_flux_kernel(VisArgs va, Ops
   const int tid = threadIdx.x, tile = blockIdx.x;
   const int ci = tid % T, fl = tid / T;
   const int fam = fl >= N ? 1 : 0, line = fl - fam * N;
-  const int cell = tile * T + ci;
-  const bool valid = tid < NT && cell < va.ncell;
+  const int cell = tile * T + ci;   // the slot
+  const bool valid = tid < NT && va.s2c[cell] >= 0;
   if (valid) {
 #pragma unroll 1
     for (int r = 0; r < NF; ++r) {

@@ -29,6 +29,11 @@ CASES = {
                       dict(N=5, dt=5e-4), "smooth"),
     "ogrid8x16_N3": (lambda: M.ogrid(8, 16, 0.5, 15.0), dict(N=3, dt=1e-3), "smooth"),
     "periodic6x5_N4": (lambda: M.periodic_square(6, 5, 2.0, 1.5), dict(N=4, dt=1e-3), "smooth"),
+    # rows of 193, 241 and 321 cells: strips padded to the mesh rows (the
+    # tiles of one row line up with the next, padding slots inside the mesh)
+    "ogrid2x193_N5": (lambda: M.ogrid(2, 193, 5.0, 6.0), dict(N=5, dt=5e-4), "smooth"),
+    "periodic241x2_N4": (lambda: M.periodic_square(241, 2, 24.1, 0.2), dict(N=4, dt=1e-3), "smooth"),
+    "ogrid2x321_N3": (lambda: M.ogrid(2, 321, 5.0, 5.6), dict(N=3, dt=5e-4), "smooth"),
 }
 
 
