@@ -105,7 +105,7 @@ static int fail(sd2d_ctx *c, int code, const std::string &m) {
 // small device kernels
 // ---------------------------------------------------------------------------
 // Fortran Q(9,N,N,NCELL) -> tile-major SoA through the slot -> cell map
-// (the padding of the last tile gets zeros; no valid lane reads it).
+// (padding slots get zeros; no valid lane reads them).
 template <int N>
 __global__ void k_f2t(const double *__restrict__ f, double *__restrict__ t,
                       const int *__restrict__ s2c, int ntile) {

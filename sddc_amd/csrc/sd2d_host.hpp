@@ -18,7 +18,7 @@ namespace sd2d {
 struct HostTables {
   int N = 0, ncell = 0, cpw = 0, ntile = 0;
   std::vector<int> c2s;      // cell -> slot (tile*cpw + position in the tile)
-  std::vector<int> s2c;      // slot -> cell, -1 for the padding of the last tile
+  std::vector<int> s2c;      // slot -> cell, -1 for a padding slot
   Ops ops{};
   Phys ph{};
   std::vector<int> ft;       // [tile][k][{neighbour base, info}][ci]

@@ -57,9 +57,9 @@ struct StageArgs {
   const double *avtab;      // hlow[isl][is] then hhigh[is][isl] (AV.f90:18-63)
   const double *visres;     // viscous divergence dFv + dGv (tile layout), M_VIS
   const int *s2c;           // slot -> mesh cell, for the cell-indexed outputs
-  // ensemble of runs (null for one run): per tile (member << 6 | valid
-  // cells); per member dt (qout then uses cr * dt), and gmax / slots hold
-  // one entry per member
+  // per tile (member << 6 | valid cells), member 0 for one run; an
+  // ensemble of runs also has per-member dt (qout then uses cr * dt, null
+  // for one run), and gmax / slots hold one entry per member
   const int *tinfo;
   const double *dtm;
   int rev;                  // walk the launch order backwards (alternate stages)
@@ -361,9 +361,9 @@ __global__ __launch_bounds__(STAGE_THREADS, 2) void stage_kernel(StageArgs a, Op
   const int line = (tid & 63) / T;     // N lines of T cells; the rest of the wave idles
   const int ci = (tid & 63) % T;
   const int cell = tile * T + ci;
-  // ensemble: the tile's member and valid-cell count; one run: member 0
-  int mem = 0, nvt = a.ncell - tile * T;
-  if (a.tinfo) { const int ti = a.tinfo[tile]; mem = ti >> 6; nvt = ti & 63; }
+  // the tile's member (0 for one run) and valid cells (its first slots)
+  const int ti = a.tinfo[tile];
+  const int mem = ti >> 6, nvt = ti & 63;
   const bool valid = line < N && ci < nvt;
   const double ge = a.gmax[2 * mem], gu = a.gmax[2 * mem + 1];
   const double cr = a.dtm ? a.cr * a.dtm[mem] : a.cr;   // b * dt of the member

@@ -77,9 +77,9 @@ void stage_kernel_mx(StageArgs a, OpsF ops, const float *__restrict__ metf) {
   const int fam = tid >> 6;            // 0: xi-line (row js), 1: eta-line (column is)
   const int line = (tid & 63) / T;     // N lines of T cells; the rest of the wave idles
   const int ci = (tid & 63) % T;
-  // ensemble: the tile's member and valid-cell count; one run: member 0
-  int mem = 0, nvt = a.ncell - tile * T;
-  if (a.tinfo) { const int ti = a.tinfo[tile]; mem = ti >> 6; nvt = ti & 63; }
+  // the tile's member (0 for one run) and valid cells (its first slots)
+  const int ti = a.tinfo[tile];
+  const int mem = ti >> 6, nvt = ti & 63;
   const bool valid = line < N && ci < nvt;
   const double ge = a.gmax[2 * mem], gu = a.gmax[2 * mem + 1];
   const double cr = a.dtm ? a.cr * a.dtm[mem] : a.cr;   // b * dt of the member
