@@ -229,8 +229,13 @@ void stage_kernel_mx(StageArgs a, OpsF ops, const float *__restrict__ metf) {
   // the velocity and the field enter the sources of every variable; the
   // energy and psi only their own update
   auto needed = [](int v, int v0, int v1) { return v < 4 || (v >= 5 && v < 8) || (v >= v0 && v < v1); };
-  double qn[9], xn[VS];
+  double qn[9], xn[VS], jis[N];
   if (valid) {
+#pragma unroll
+    for (int s = 0; s < N; ++s) {   // 1/J at the thread's points
+      const int is = fam ? line : s, js = fam ? s : line;
+      jis[s] = mt[(G::JI + js + N * is) * T];
+    }
     const int v0 = fam ? VS : 0, v1 = fam ? 9 : VS;
 #pragma unroll
     for (int v = 0; v < 9; ++v)
@@ -287,8 +292,7 @@ void stage_kernel_mx(StageArgs a, OpsF ops, const float *__restrict__ metf) {
 #pragma unroll
     for (int s = 0; s < N; ++s) {
       const int p = p0 + ps * s;
-      const int is = fam ? line : s, js = fam ? s : line;
-      const double ji = mt[(G::JI + js + N * is) * T];
+      const double ji = jis[s];
       double R[9], q[9], xc[VS];
 #pragma unroll
       for (int v = 0; v < 9; ++v)
