@@ -768,8 +768,8 @@ __global__ __launch_bounds__(STAGE_THREADS, 2) void stage_kernel(StageArgs a, Op
     double *jz2 = lds + LD::A_JZ;            // [p][ci]
     double *lowb = lds + LD::A_LOW;          // [2][NL2][ci]
     double *seb = lds + LD::A_SE;            // [2][N2][ci]
-    // the state area is free: every xi-line thread has read its points
-    // before the reduction barrier above
+    // the state area is free: every thread has read its points before the
+    // reduction barrier above
     if (valid && fam == 0) {
 #pragma unroll
       for (int s = 0; s < N; ++s) {
