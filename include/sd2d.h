@@ -50,7 +50,9 @@ enum { SD2D_FP64 = 0, SD2D_MIXED = 1 };
 
 /* Tile shape: the device processes cells in tiles of 12/16/20 (N = 5/4/3),
  * one workgroup per tile.  SD2D_TILE_STRIPS takes consecutive cells of the
- * mesh numbering; SD2D_TILE_BLOCKS groups compact patches of cells
+ * mesh numbering (on a mesh numbered row by row, each row padded to a
+ * multiple of the tile when that adds at most 1/16, so that the tiles of
+ * consecutive rows line up); SD2D_TILE_BLOCKS groups compact patches of cells
  * (recursive bisection of the cell adjacency graph), so fewer neighbour
  * traces come from other tiles, but those that do are scattered over more
  * tiles.  SD2D_TILE_DEFAULT is SD2D_TILE_STRIPS (the faster of the two on
