@@ -427,7 +427,7 @@ static void inviscid_flux_interior(orc_ctx *o, double *umax) {
 }
 
 /* One face with two cells: interfaceflux (interfaceflux_all.f90:16-110)
- * and CYCLOCFLUX (cyclocflux_all.f90:15-305) differ only in how the cells
+ * and CYCLOCFLUX (cyclocflux_all.f90:15-110) differ only in how the cells
  * and local faces are looked up. */
 static void two_cell_face(orc_ctx *o, int icleft, int icright, int ifacelc, int ifacerc,
                           double *eig, double *um) {

@@ -47,6 +47,7 @@ struct sd2d_ctx {
   int *c2s = nullptr, *s2c = nullptr;          // cell <-> slot maps of the tile shape
   double *avtab = nullptr;
   double *gmax = nullptr;                       // eigvmax_g, umax_g
+  double *lastch = nullptr;                     // c_h of the last evaluation (per member)
   unsigned long long *slots = nullptr;
   double *sm_ind = nullptr;
   double *normbuf = nullptr, *normout = nullptr;
@@ -58,6 +59,8 @@ struct sd2d_ctx {
   int nmem = 1;
   long long nlaunch = 0;                // stage launches of sd2d_step so far
   int next_stage = 1;                   // sd2d_rk_stage: the next stage of the step
+  int check_every = 0;                  // opt-in non-finite check (sd2d_set_check_finite)
+  int *nfflag = nullptr;                // its per-member flags (device)
   // the line family (0: xi, 1: eta) whose wave stages a tile into LDS: the
   // one with fewer end faces whose neighbour lies in another tile (fewer
   // neighbour gathers), from the face table
@@ -72,7 +75,6 @@ struct sd2d_ctx {
   double *gq = nullptr, *dei = nullptr, *visres = nullptr, *metw = nullptr;
   double ctime = 0.0;
   int iter = 0;
-  double last_c_h = 0.0;
   double resid_rms = 0.0, divb_rms = 0.0;
   int norms_valid = 0;
   bool have_av = false;
@@ -86,6 +88,26 @@ struct sd2d_ctx {
 };
 
 static std::string g_err = "no error";
+
+// vismode 0 is refused: the reference then forms the Powell source and the
+// divB norm from a nablaQs it never computes (compflux.f90:13-15 against
+// compresid.f90:58, iterations.f90:43), so its result is undefined.
+static const char *const vismode_msg =
+    "vismode must be 1: with vismode 0 the reference reads nablaQs without computing it "
+    "(compflux.f90:13-15, compresid.f90:58), so there is no defined result to reproduce";
+
+// Calls that read or replace the state, or evaluate the residual outside
+// the stage sequence, are refused between sd2d_rk_stage calls of one step:
+// the state buffer then holds a stage's working values (RK_S5P3 stage 4
+// writes its output in place) and a residual evaluation would raise the
+// running maxima in the middle of the step.
+#define SD2D_NOT_MID_STEP(c, what)                                                   \
+  do {                                                                           \
+    if ((c)->next_stage != 1)                                                    \
+      return fail((c), SD2D_EINVAL,                                              \
+                  std::string(what) + ": a step is in progress (sd2d_rk_stage): the next stage is " + \
+                      std::to_string((c)->next_stage));                          \
+  } while (0)
 
 #define HIPCHK(ctx, call)                                                       \
   do {                                                                          \
@@ -139,13 +161,22 @@ __global__ void k_t2f(const double *__restrict__ t, double *__restrict__ f,
   }
 }
 
-// running maxima: g = max over slots (one block per ensemble member)
+// running maxima: g = max over slots (one block per ensemble member).
+// The values g holds on entry are the ones the stage just finished formed
+// its c_h from (compflux.f90:6), so that c_h is recorded first, with the
+// kernel's own formula.  No NaN reaches the slots: every point adopts a
+// speed only through `speed > running max`, which a NaN never passes
+// (getrusanov.f90:116-123).
 __global__ void k_finalize_max(const unsigned long long *__restrict__ slots,
-                               double *__restrict__ gmax) {
+                               double *__restrict__ gmax, double *__restrict__ lastch) {
   __shared__ unsigned long long s[2][NSLOT];
   const int t = threadIdx.x;
   slots += 2 * NSLOT * (size_t)blockIdx.x;
   gmax += 2 * blockIdx.x;
+  if (t == 0) {
+    const double ge = gmax[0], gu = gmax[1];
+    lastch[blockIdx.x] = sqrt(ge * (ge - gu));
+  }
   s[0][t] = slots[2 * t];
   s[1][t] = slots[2 * t + 1];
   __syncthreads();
@@ -159,6 +190,22 @@ __global__ void k_finalize_max(const unsigned long long *__restrict__ slots,
   if (t == 0) {
     gmax[0] = __longlong_as_double((long long)s[0][0]);
     gmax[1] = __longlong_as_double((long long)s[1][0]);
+  }
+}
+
+// Non-finite scan of a state in tile layout (the opt-in check of
+// sd2d_set_check_finite): flags[member] = 1 when a tile of that member holds
+// a NaN or an infinity.  Padding slots hold zeros, so they never raise it.
+__global__ void k_scan_finite(const double2 *__restrict__ q, const int *__restrict__ tinfo,
+                              int ntile, int qt2, int *__restrict__ flags) {
+  for (int t = blockIdx.x; t < ntile; t += gridDim.x) {   // uniform over the block
+    const double2 *b = q + (size_t)t * qt2;
+    bool bad = false;
+    for (int i = threadIdx.x; i < qt2; i += blockDim.x) {
+      const double2 v = b[i];
+      bad |= !(isfinite(v.x) && isfinite(v.y));
+    }
+    if (__syncthreads_or(bad) && threadIdx.x == 0) atomicOr(&flags[tinfo[t] >> 6], 1);
   }
 }
 
@@ -330,8 +377,35 @@ static StageArgs base_args(sd2d_ctx *c) {
   return a;
 }
 
+// The opt-in finiteness scan of the state in q[c->cur]: flags (c->nmem
+// entries, host) get 1 for every member whose state holds a NaN or an
+// infinity.
+static int scan_finite(sd2d_ctx *c, int *flags) {
+  const int qt2 = 9 * c->N * c->N * c->cpw / 2;
+  HIPCHK(c, hipMemsetAsync(c->nfflag, 0, (size_t)c->nmem * sizeof(int), c->stream));
+  hipLaunchKernelGGL(k_scan_finite, dim3(std::min(c->ntile, 8192)), dim3(256), 0, c->stream,
+                     reinterpret_cast<const double2 *>(c->q[c->cur]), c->tinfo, c->ntile, qt2,
+                     c->nfflag);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(flags, c->nfflag, (size_t)c->nmem * sizeof(int), hipMemcpyDeviceToHost,
+                           c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return SD2D_OK;
+}
+
+static int check_finite_now(sd2d_ctx *c) {
+  int bad = 0;
+  const int r = scan_finite(c, &bad);
+  if (r) return r;
+  if (bad)
+    return fail(c, SD2D_ENONFINITE,
+                "non-finite value (NaN or Inf) in the state after step " + std::to_string(c->iter));
+  return SD2D_OK;
+}
+
 static int finalize_stage(sd2d_ctx *c) {
-  hipLaunchKernelGGL(k_finalize_max, dim3(c->nmem), dim3(NSLOT), 0, c->stream, c->slots, c->gmax);
+  hipLaunchKernelGGL(k_finalize_max, dim3(c->nmem), dim3(NSLOT), 0, c->stream, c->slots, c->gmax,
+                     c->lastch);
   HIPCHK(c, hipGetLastError());
   return SD2D_OK;
 }
@@ -390,6 +464,9 @@ static int ctx_upload(sd2d_ctx *c, const HostTables &tb, int nwall, int nmem,
   CK(hipMalloc(&c->s2c, tb.s2c.size() * sizeof(int)));
   CK(hipMalloc(&c->avtab, av.size() * sizeof(double)));
   CK(hipMalloc(&c->gmax, 2 * (size_t)nmem * sizeof(double)));
+  CK(hipMalloc(&c->lastch, (size_t)nmem * sizeof(double)));
+  CK(hipMalloc(&c->nfflag, (size_t)nmem * sizeof(int)));
+  CK(hipMemsetAsync(c->lastch, 0, (size_t)nmem * sizeof(double), c->stream));
   CK(hipMalloc(&c->slots, 2 * NSLOT * (size_t)nmem * sizeof(unsigned long long)));
   CK(hipMalloc(&c->sm_ind, 2 * (size_t)nc * sizeof(double)));
   CK(hipMalloc(&c->normbuf, 2 * (size_t)nt * sizeof(double)));
@@ -461,6 +538,8 @@ int sd2d_create(const sd2d_mesh *m, const sd2d_params *p, sd2d_ctx **out) {
     return fail(nullptr, SD2D_EINVAL, "incomplete mesh");
   if (p->k_stage != 3 && p->k_stage != 5)
     return fail(nullptr, SD2D_EINVAL, "k_stage can be only 3 or 5");
+  if (p->vismode != 1)
+    return fail(nullptr, SD2D_EUNSUPPORTED, vismode_msg);
   if (p->precision != SD2D_FP64 && p->precision != SD2D_MIXED)
     return fail(nullptr, SD2D_EINVAL, "precision must be SD2D_FP64 or SD2D_MIXED");
   const bool vis = p->vismode == 1 && (p->mu != 0.0 || p->eta0 != 0.0);
@@ -503,7 +582,7 @@ void sd2d_destroy(sd2d_ctx *c) {
   adj_free(c->adj);
   for (double *b : c->q) if (b) (void)hipFree(b);
   for (double *b : c->fsr) if (b) (void)hipFree(b);
-  void *ptrs[] = {c->met, c->metf, c->ftab, c->avtab, c->gmax, c->slots, c->sm_ind, c->normbuf,
+  void *ptrs[] = {c->met, c->metf, c->ftab, c->avtab, c->gmax, c->lastch, c->nfflag, c->slots, c->sm_ind, c->normbuf,
                   c->normout, c->stagef, c->nablaf, c->gq, c->dei, c->visres, c->metw,
                   c->c2s, c->s2c, c->tinfo, c->dtm, c->phm};
   for (void *b : ptrs) if (b) (void)hipFree(b);
@@ -554,6 +633,7 @@ int sd2d_set_state(sd2d_ctx *c, const double *q, const sd2d_maxima *mx) {
 
 int sd2d_get_state(sd2d_ctx *c, double *q, sd2d_maxima *mx) {
   if (!c) return fail(nullptr, SD2D_EINVAL, "ctx is NULL");
+  if (q) SD2D_NOT_MID_STEP(c, "sd2d_get_state");
   const size_t n = (size_t)c->ncell * 9 * c->N * c->N;
   if (q) {
     int r = t2f(c, c->q[c->cur], c->stagef);
@@ -583,20 +663,12 @@ int sd2d_set_time(sd2d_ctx *c, double ctime, int32_t iter) {
   return SD2D_OK;
 }
 
-static int read_c_h(sd2d_ctx *c) {
-  double g[2];
-  HIPCHK(c, hipMemcpyAsync(g, c->gmax, sizeof(g), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
-  c->last_c_h = std::sqrt(g[0] * (g[0] - g[1]));
-  return SD2D_OK;
-}
-
 int sd2d_compresid(sd2d_ctx *c, double *resid_out, double *nablaqs_out) {
   if (!c) return fail(nullptr, SD2D_EINVAL, "ctx is NULL");
+  SD2D_NOT_MID_STEP(c, "sd2d_compresid");
   const size_t n = (size_t)c->ncell * 9 * c->N * c->N;
   if (!c->nablaf) HIPCHK(c, hipMalloc(&c->nablaf, 2 * n * sizeof(double)));
-  int r = read_c_h(c);
-  if (r) return r;
+  int r;
   StageArgs a = base_args(c);
   a.qin = c->q[c->cur];
   a.resid_f = c->stagef;
@@ -667,7 +739,11 @@ int sd2d_rk_stage(sd2d_ctx *c, int32_t stage) {
   c->next_stage = stage == 5 ? 1 : stage + 1;
   if (stage == 5) c->have_av = c->p.artiv == 1 && c->p.vismode == 1;
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  if (c->prof) return collect_profile(c);
+  if (c->prof) {
+    const int rp = collect_profile(c);
+    if (rp) return rp;
+  }
+  if (stage == 5 && c->check_every > 0 && c->iter % c->check_every == 0) return check_finite_now(c);
   return SD2D_OK;
 }
 
@@ -699,6 +775,10 @@ int sd2d_step(sd2d_ctx *c, int32_t nsteps, int32_t norm_every) {
       c->divb_rms = std::sqrt(s2[1] / nn);
       c->norms_valid = 1;
     }
+    if (c->check_every > 0 && (c->iter % c->check_every == 0 || s == nsteps - 1)) {
+      r = check_finite_now(c);
+      if (r) return r;
+    }
   }
   HIPCHK(c, hipStreamSynchronize(c->stream));
   if (c->prof) return collect_profile(c);
@@ -707,6 +787,7 @@ int sd2d_step(sd2d_ctx *c, int32_t nsteps, int32_t norm_every) {
 
 int sd2d_write_restart(sd2d_ctx *c, const char *path, const int32_t *icg) {
   if (!c || !path) return fail(c, SD2D_EINVAL, "NULL argument");
+  SD2D_NOT_MID_STEP(c, "sd2d_write_restart");
   const size_t rec = (size_t)9 * c->N * c->N;
   std::vector<double> q(rec * c->ncell);
   int r = sd2d_get_state(c, q.data(), nullptr);
@@ -763,6 +844,13 @@ int sd2d_read_restart(sd2d_ctx *c, const char *path) {
   return sd2d_set_time(c, t, it);
 }
 
+int sd2d_set_check_finite(sd2d_ctx *c, int32_t every) {
+  if (!c) return fail(nullptr, SD2D_EINVAL, "ctx is NULL");
+  if (every < 0) return fail(c, SD2D_EINVAL, "every must be >= 0");
+  c->check_every = every;
+  return SD2D_OK;
+}
+
 int sd2d_set_profiling(sd2d_ctx *c, int32_t on) {
   if (!c) return fail(nullptr, SD2D_EINVAL, "ctx is NULL");
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -785,14 +873,15 @@ int sd2d_get_profile(sd2d_ctx *c, double *stage_ms, int32_t *nlaunch) {
 
 int sd2d_get_diag(sd2d_ctx *c, sd2d_diag *d) {
   if (!c || !d) return fail(c, SD2D_EINVAL, "NULL argument");
-  double g[2];
+  double g[2], ch = 0.0;
   HIPCHK(c, hipMemcpyAsync(g, c->gmax, sizeof(g), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(&ch, c->lastch, sizeof(ch), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   d->resid_rms = c->resid_rms;
   d->divb_rms = c->divb_rms;
   d->eigvmax_g = g[0];
   d->umax_g = g[1];
-  d->c_h = c->last_c_h;
+  d->c_h = ch;
   const double h = std::sqrt(c->p.dxcycl * c->p.dycycl / (double)c->p.ncellglob);
   d->epsilon0 = g[0] * h / (double)(c->N - 1) * 0.05;
   d->ctime = c->ctime;

@@ -376,6 +376,7 @@ int sd2d_fwd(sd2d_ctx *c, int32_t nsteps, int32_t ckpt, double *objective) {
 int sd2d_vjp(sd2d_ctx *c, const double *qbar_final, double jbar, double *qbar_init,
              double *pbar, double *chbar) {
   if (!c) return fail(nullptr, SD2D_EINVAL, "ctx is NULL");
+  SD2D_NOT_MID_STEP(c, "sd2d_vjp");
   AdjState *A = c->adj;
   if (!A || !A->have_fwd) return fail(c, SD2D_EINVAL, "sd2d_vjp needs a preceding sd2d_fwd");
   const int nsteps = A->nsteps, ckpt = A->ckpt, nck = (nsteps + ckpt - 1) / ckpt + 1;
@@ -435,6 +436,7 @@ int sd2d_vjp(sd2d_ctx *c, const double *qbar_final, double jbar, double *qbar_in
 
 int sd2d_compresid_vjp(sd2d_ctx *c, const double *rbar, double *qbar, double *chbar, double *pbar) {
   if (!c || !rbar) return fail(c, SD2D_EINVAL, "NULL argument");
+  SD2D_NOT_MID_STEP(c, "sd2d_compresid_vjp");
   if (c->p.precision != SD2D_FP64 || c->vis || c->nmem > 1)
     return fail(c, SD2D_EUNSUPPORTED, "the adjoint covers the FP64 inviscid path");
   if (!c->adj) c->adj = new (std::nothrow) AdjState();

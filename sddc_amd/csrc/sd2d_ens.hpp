@@ -34,6 +34,8 @@ struct sd2d_ens {
   std::vector<hipEvent_t> join;
   hipGraphExec_t graph[3] = {nullptr, nullptr, nullptr};   // by rotation position
   int cur = 0;                        // state buffer holding every member's state
+  int check_every = 0;                // opt-in non-finite check
+  std::vector<int> nonfinite;         // per member, from the latest check
   std::string err;
 };
 
@@ -269,6 +271,7 @@ int sd2d_ens_create(int32_t nmem, const sd2d_mesh *meshes, const sd2d_params *pa
       return ens_fail(nullptr, SD2D_EINVAL, w + "incomplete mesh");
     if (p.k_stage != 3)
       return ens_fail(nullptr, SD2D_EUNSUPPORTED, w + "ensembles step with RK_S5P3 (k_stage 3)");
+    if (p.vismode != 1) return ens_fail(nullptr, SD2D_EUNSUPPORTED, w + vismode_msg);
     if (p.vismode == 1 && (p.mu != 0.0 || p.eta0 != 0.0))
       return ens_fail(nullptr, SD2D_EUNSUPPORTED, w + "ensembles run the inviscid path");
     if (p.precision != params[0].precision || p.artiv != params[0].artiv ||
@@ -348,6 +351,29 @@ int sd2d_ens_get_time(sd2d_ens *e, int32_t i, double *ctime, int32_t *iter) {
   return SD2D_OK;
 }
 
+// The opt-in finiteness scan of every member's state (after a step):
+// e->nonfinite[i] = 1 for the members holding a NaN or an infinity.
+static int ens_check_finite(sd2d_ens *e, int steps_done) {
+  bool any = false;
+  e->nonfinite.assign(e->nmem, 0);
+  for (size_t g = 0; g < e->grp.size(); ++g) {
+    sd2d_ctx *c = e->grp[g];
+    c->cur = e->cur;
+    std::vector<int> fl(c->nmem, 0);
+    const int r = scan_finite(c, fl.data());
+    if (r) { e->err = c->err; return r; }
+    for (int i = 0; i < e->nmem; ++i)
+      if (e->mem[i].group == (int)g && fl[e->mem[i].local]) { e->nonfinite[i] = 1; any = true; }
+  }
+  if (!any) return SD2D_OK;
+  std::string who;
+  for (int i = 0; i < e->nmem; ++i)
+    if (e->nonfinite[i]) who += (who.empty() ? "" : ", ") + std::to_string(i);
+  return ens_fail(e, SD2D_ENONFINITE,
+                  "non-finite value (NaN or Inf) in the state of member(s) " + who + " after " +
+                      std::to_string(steps_done) + " step(s) of this call");
+}
+
 int sd2d_ens_step(sd2d_ens *e, int32_t nsteps) {
   if (!e) return ens_fail(nullptr, SD2D_EINVAL, "ens is NULL");
   if (nsteps < 0) return ens_fail(e, SD2D_EINVAL, "nsteps < 0");
@@ -362,9 +388,28 @@ int sd2d_ens_step(sd2d_ens *e, int32_t nsteps) {
       m.ctime = m.ctime + e->params[&m - e->mem.data()].dt;
       m.iter += 1;
     }
+    if (e->check_every > 0 && ((s + 1) % e->check_every == 0 || s == nsteps - 1)) {
+      for (sd2d_ctx *c : e->grp) c->cur = e->cur;
+      const int r = ens_check_finite(e, s + 1);
+      if (r) return r;
+    }
   }
   for (sd2d_ctx *c : e->grp) c->cur = e->cur;
   ENSCHK(e, hipStreamSynchronize(e->stream));
+  return SD2D_OK;
+}
+
+int sd2d_ens_set_check_finite(sd2d_ens *e, int32_t every) {
+  if (!e) return ens_fail(nullptr, SD2D_EINVAL, "ens is NULL");
+  if (every < 0) return ens_fail(e, SD2D_EINVAL, "every must be >= 0");
+  e->check_every = every;
+  return SD2D_OK;
+}
+
+int sd2d_ens_nonfinite(sd2d_ens *e, int32_t *flags) {
+  if (!e || !flags) return ens_fail(e, SD2D_EINVAL, "NULL argument");
+  for (int i = 0; i < e->nmem; ++i)
+    flags[i] = i < (int)e->nonfinite.size() ? e->nonfinite[i] : 0;
   return SD2D_OK;
 }
 

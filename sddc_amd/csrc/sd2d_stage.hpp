@@ -303,7 +303,7 @@ __device__ __forceinline__ void face_trace_lds(const R *qs, const OP &ops, int i
 
 // Flux at an end face point: Rusanov in the reference's left/right roles
 // with the gathered neighbour trace (interfaceflux_all.f90:31-106,
-// cyclocflux_all.f90:226-301) or with the boundary ghost (bcflux.f90).
+// cyclocflux_all.f90:15-110) or with the boundary ghost (bcflux.f90).
 // Returns the flux stored for this cell and the averaged state Qv.
 template <int N, typename R>
 __device__ __forceinline__ void face_flux(int info, int fam, int rfp, const Phys &ph,

@@ -173,6 +173,13 @@ module sd2d_c
        integer(c_int32_t), intent(out), optional :: nlaunch
      end function sd2d_get_profile
 
+     ! opt-in NaN/Inf check of the state every `every` steps (0: off)
+     integer(c_int) function sd2d_set_check_finite(ctx, every) bind(C, name="sd2d_set_check_finite")
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: every
+     end function sd2d_set_check_finite
+
      ! ensembles: meshes(nmem), params(nmem) arrays of the derived types
      integer(c_int) function sd2d_ens_create(nmem, meshes, params, ens) &
           bind(C, name="sd2d_ens_create")
@@ -243,6 +250,20 @@ module sd2d_c
        integer(c_int32_t), value :: trial_steps
        integer(c_int32_t), intent(inout), optional :: shapes(*)
      end function sd2d_ens_tune
+
+     integer(c_int) function sd2d_ens_set_check_finite(ens, every) &
+          bind(C, name="sd2d_ens_set_check_finite")
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: ens
+       integer(c_int32_t), value :: every
+     end function sd2d_ens_set_check_finite
+
+     ! flags(nmem): 1 for the members the latest check found non-finite
+     integer(c_int) function sd2d_ens_nonfinite(ens, flags) bind(C, name="sd2d_ens_nonfinite")
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: ens
+       integer(c_int32_t), intent(out) :: flags(*)
+     end function sd2d_ens_nonfinite
 
      ! adjoint: forward with checkpoints, then the vector-Jacobian product
      integer(c_int) function sd2d_fwd(ctx, nsteps, ckpt, objective) bind(C, name="sd2d_fwd")

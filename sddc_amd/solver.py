@@ -31,11 +31,16 @@ EXPORTS = ("sd2d_create", "sd2d_destroy", "sd2d_last_error", "sd2d_set_state",
            "sd2d_ens_create", "sd2d_ens_destroy", "sd2d_ens_last_error",
            "sd2d_ens_set_state", "sd2d_ens_get_state", "sd2d_ens_set_time",
            "sd2d_ens_get_time", "sd2d_ens_step", "sd2d_ens_tune",
+           "sd2d_set_check_finite", "sd2d_ens_set_check_finite", "sd2d_ens_nonfinite",
            "sd2d_fwd", "sd2d_vjp", "sd2d_compresid_vjp")
 
 
 class Sd2dError(RuntimeError):
     pass
+
+
+class Sd2dNonFinite(Sd2dError):
+    """The opt-in finiteness check found a NaN or an infinity."""
 
 
 def load_library(path: str = LIBPATH):
@@ -79,6 +84,9 @@ def load_library(path: str = LIBPATH):
                                     C.POINTER(C.c_int32)]
     L.sd2d_ens_step.argtypes = [vp, C.c_int32]
     L.sd2d_ens_tune.argtypes = [vp, C.c_int32, vp]
+    L.sd2d_set_check_finite.argtypes = [vp, C.c_int32]
+    L.sd2d_ens_set_check_finite.argtypes = [vp, C.c_int32]
+    L.sd2d_ens_nonfinite.argtypes = [vp, vp]
     L.sd2d_fwd.argtypes = [vp, C.c_int32, C.c_int32, C.POINTER(C.c_double)]
     L.sd2d_vjp.argtypes = [vp, vp, C.c_double, vp, vp, vp]
     L.sd2d_compresid_vjp.argtypes = [vp, vp, vp, C.POINTER(C.c_double), vp]
@@ -113,7 +121,8 @@ class Sd2d:
 
     def _check(self, rc, what):
         if rc != 0:
-            raise Sd2dError(f"{what}: {rc}: {self.L.sd2d_last_error(self.h).decode()}")
+            cls = Sd2dNonFinite if rc == abi.SD2D_ENONFINITE else Sd2dError
+            raise cls(f"{what}: {rc}: {self.L.sd2d_last_error(self.h).decode()}")
 
     def close(self):
         if getattr(self, "h", None):
@@ -143,6 +152,13 @@ class Sd2d:
     def set_time(self, ctime, it=0):
         self._check(self.L.sd2d_set_time(self.h, float(ctime), int(it)), "sd2d_set_time")
 
+    def maxima(self):
+        """The running maxima alone [eigvmax, umax, eigvmax_g, umax_g]
+        (allowed between the stages of a step)."""
+        mx = abi.Sd2dMaxima()
+        self._check(self.L.sd2d_get_state(self.h, None, C.byref(mx)), "sd2d_get_state")
+        return np.array([mx.eigvmax, mx.umax, mx.eigvmax_g, mx.umax_g])
+
     # -- the path ----------------------------------------------------------
     def compresid(self, want_resid=True, want_nabla=True):
         r = np.zeros((9, self.N, self.N, self.nc), order="F") if want_resid else None
@@ -171,6 +187,11 @@ class Sd2d:
     def read_restart(self, path):
         """Load a reference-format restart file (maxima reset to 0)."""
         self._check(self.L.sd2d_read_restart(self.h, str(path).encode()), "sd2d_read_restart")
+
+    def set_check_finite(self, every: int):
+        """Opt-in NaN/Inf check of the state every `every` steps (0: off);
+        a hit makes step() raise Sd2dNonFinite."""
+        self._check(self.L.sd2d_set_check_finite(self.h, int(every)), "sd2d_set_check_finite")
 
     def set_profiling(self, on: bool):
         self._check(self.L.sd2d_set_profiling(self.h, int(bool(on))), "sd2d_set_profiling")
@@ -243,7 +264,8 @@ class Sd2dEnsemble:
 
     def _check(self, rc, what):
         if rc != 0:
-            raise Sd2dError(f"{what}: {rc}: {self.L.sd2d_ens_last_error(self.h).decode()}")
+            cls = Sd2dNonFinite if rc == abi.SD2D_ENONFINITE else Sd2dError
+            raise cls(f"{what}: {rc}: {self.L.sd2d_ens_last_error(self.h).decode()}")
 
     def close(self):
         if getattr(self, "h", None):
@@ -280,6 +302,18 @@ class Sd2dEnsemble:
 
     def step(self, nsteps):
         self._check(self.L.sd2d_ens_step(self.h, int(nsteps)), "sd2d_ens_step")
+
+    def set_check_finite(self, every: int):
+        """Opt-in NaN/Inf check of every member every `every` steps (0: off);
+        a hit makes step() raise Sd2dNonFinite, nonfinite() names them."""
+        self._check(self.L.sd2d_ens_set_check_finite(self.h, int(every)),
+                    "sd2d_ens_set_check_finite")
+
+    def nonfinite(self):
+        """Per member: 1 when the latest check found a NaN or an infinity."""
+        f = np.zeros(len(self.cases), dtype=np.int32)
+        self._check(self.L.sd2d_ens_nonfinite(self.h, _ptr(f)), "sd2d_ens_nonfinite")
+        return f
 
     def tune(self, trial_steps=3):
         """Autotune the tile shape of every order present; returns

@@ -24,6 +24,12 @@ SPECS = {
     "viscous_channel6x5_N4": (dict(kind="channel", nx=6, ny=5, seed=7, free_left=2),
                               dict(N=4, dt=5e-4, mu=0.01, eta0=0.02),
                               dict(kind="smooth", amp=0.05, seed=2), 1, 2),
+    # the viscous/resistive path on a periodic mesh: BR1 gradient averaging
+    # across the CYCL pairs (cyclocflux_all.f90:114-204); 5 x 4 cells so that
+    # the x- and y-pairs differ in number
+    "viscous_periodic5x4_N3": (dict(kind="periodic", nx=5, ny=4),
+                               dict(N=3, dt=5e-4, mu=0.01, eta0=0.02),
+                               dict(kind="smooth", amp=0.05, seed=3), 2, 2),
     "fsrk_ogrid4x8_N4": (dict(kind="ogrid", nr=4, nt=8, r0=0.5, r1=15.0),
                          dict(N=4, dt=1e-3, k_stage=5), dict(kind="smooth", amp=0.05, seed=4), 0, 3),
     "c1_cylinder": (dict(kind="ogrid", nr=32, nt=64, r0=0.5, r1=15.0),

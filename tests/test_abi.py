@@ -117,6 +117,34 @@ def test_viscous_mixed_rejected():
     assert rc == abi.SD2D_EUNSUPPORTED and "viscous" in msg
 
 
+@pytest.mark.parametrize("vismode", [0, 2])
+def test_vismode_other_than_1_rejected(vismode):
+    """vismode 0 has no defined result in the reference (its Powell source
+    and divB norm read a nablaQs that is never computed, compflux.f90:13-15
+    against compresid.f90:58), so it is refused, as is any value but 1."""
+    rc, h, msg = _create(_small_case(), mutate_params=lambda p: setattr(p, "vismode", vismode))
+    assert rc == abi.SD2D_EUNSUPPORTED and "vismode" in msg
+    assert h.value is None
+    L = _lib()
+    case = _small_case()
+    ms, keep = abi.mesh_struct(case)
+    ps = abi.params_struct(case.params, case.ncell)
+    ps.vismode = vismode
+    meshes = (abi.Sd2dMesh * 1)(ms)
+    params = (abi.Sd2dParams * 1)(ps)
+    e = C.c_void_p(99)
+    rc = L.sd2d_ens_create(1, meshes, params, C.byref(e))
+    assert rc == abi.SD2D_EUNSUPPORTED and "vismode" in L.sd2d_ens_last_error(None).decode()
+    assert e.value is None
+
+
+def test_check_finite_arguments():
+    L = _lib()
+    assert L.sd2d_set_check_finite(None, 1) == abi.SD2D_EINVAL
+    assert L.sd2d_ens_set_check_finite(None, 1) == abi.SD2D_EINVAL
+    assert L.sd2d_ens_nonfinite(None, None) == abi.SD2D_EINVAL
+
+
 def test_unknown_precision():
     rc, _, msg = _create(_small_case(), precision=7)
     assert rc == abi.SD2D_EINVAL and "precision" in msg

@@ -1,0 +1,188 @@
+"""The benchmark configurations at their own sizes, against the oracle (the
+C restatement of the reference, bit-exact to it on the golden fixtures).
+
+  C2  512 x 1024 O-grid cylinder, N=5: two sd2d_compresid calls (c_h = 0,
+      then lagged from the first call's maxima) against Oracle.compresid,
+      per variable ||r_gpu - r_oracle||_inf / ||r_oracle||_inf <= 1e-12 for
+      resid and nablaQs (compresid.f90:1-80), maxima to 1e-14 relative.
+  C3  1024 x 1024 periodic Orszag-Tang, N=4: the same two calls and one
+      RK_S5P3 step (state increment to 1e-11); mixed precision against FP64
+      on the device over 50 steps, relative L2 drift <= 1e-6 per variable.
+  C4  the 64-member ensemble (seed 4) for two steps in FP64: one member of
+      each order against the oracle run of that member (increment to 1e-11,
+      maxima to 1e-13); every member finite.
+  C5  the 128 x 256 O-grid cylinder, N=4: gradients of the time-averaged
+      wall drag with respect to Mach and field angle, and the VJP of a
+      seeded N(0,1) cotangent on the final state, against central
+      differences (h = 1e-6) of the device forward, to 1e-5.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from tests.util import per_var_error
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def _compresid_pair(case, Q0, tag):
+    """Two compresid calls on device and oracle from the same state."""
+    from sddc_amd.solver import Sd2d
+    from oracle.oracle import Oracle
+    g = Sd2d(case)
+    o = Oracle(case)
+    g.set_state(Q0, [0, 0, 0, 0])
+    o.set_state(Q0, [0, 0, 0, 0])
+    for it in range(2):
+        print(f"[gpu-configs] {tag} compresid call {it + 1}", flush=True)
+        rg, ng = g.compresid()
+        ro, no = o.compresid()
+        e = per_var_error(rg, ro)[0]
+        e2 = per_var_error(ng.reshape(18, -1, order="F"), no.reshape(18, -1, order="F"))[0]
+        print(f"[gpu-configs] {tag} call {it + 1}: resid {e.max():.2e} nablaQs {e2.max():.2e}",
+              flush=True)
+        assert np.all(np.isfinite(rg))
+        assert e.max() <= TOL, (tag, it, e)
+        assert e2.max() <= TOL, (tag, it, e2)
+        _, mg = g.get_state()
+        _, mo = o.get_state()
+        assert np.allclose(mg, mo, rtol=1e-14, atol=0), (tag, it, mg, mo)
+        del rg, ng, ro, no
+    assert g.diag()["c_h"] > 0.0
+    return g, o
+
+
+def test_c2_compresid_matches_oracle():
+    from sddc_amd import cases
+    print("[gpu-configs] building C2 (512x1024, N=5)", flush=True)
+    case, Q0 = cases.c2_cylinder()
+    g, o = _compresid_pair(case, Q0, "C2")
+    g.close()
+    o.close()
+
+
+def test_c3_orszag_tang_against_oracle():
+    from sddc_amd import abi, cases
+    from sddc_amd.solver import Sd2d
+    print("[gpu-configs] building C3 (1024x1024 periodic, N=4)", flush=True)
+    case, Q0 = cases.orszag_tang(1024, N=4)
+    g, o = _compresid_pair(case, Q0, "C3")
+    # one RK_S5P3 step from the initial state with fresh maxima
+    print("[gpu-configs] C3 one step", flush=True)
+    g.set_state(Q0, [0, 0, 0, 0]); o.set_state(Q0, [0, 0, 0, 0])
+    g.step(1, 1); o.step(1, 1)
+    Qg, mg = g.get_state()
+    Qo, mo = o.get_state()
+    e = per_var_error(Qg - Q0, Qo - Q0)[0]
+    print(f"[gpu-configs] C3 step increment error {e.max():.2e}", flush=True)
+    assert e.max() <= 1e-11, e
+    assert np.allclose(mg, mo, rtol=1e-13, atol=0), (mg, mo)
+    dg, do = g.diag(), o.diag()
+    np.testing.assert_allclose([dg["resid_rms"], dg["divb_rms"]],
+                               [do["resid_rms"], do["divb_rms"]], rtol=1e-10)
+    g.close()
+    o.close()
+    del Qo
+    # mixed against FP64 on the device over 50 steps
+    out = {}
+    for prec in (abi.SD2D_FP64, abi.SD2D_MIXED):
+        print(f"[gpu-configs] C3 50 steps precision {prec}", flush=True)
+        d = Sd2d(case, precision=prec)
+        d.set_state(Q0, [0, 0, 0, 0])
+        d.step(50)
+        out[prec] = d.get_state()[0]
+        d.close()
+    a = out[abi.SD2D_MIXED].reshape(9, -1, order="F")
+    b = out[abi.SD2D_FP64].reshape(9, -1, order="F")
+    nb = np.sqrt((b * b).sum(axis=1))
+    den = nb.copy()
+    den[[3, 7, 8]] = nb.max()          # w, Bz, psi stay zero in 2-D
+    l2 = np.sqrt(((a - b) ** 2).sum(axis=1)) / den
+    print(f"[gpu-configs] C3 mixed-vs-FP64 relative L2 after 50 steps: "
+          f"{np.array2string(l2, precision=2)}", flush=True)
+    assert np.all(np.isfinite(a))
+    assert l2.max() <= 1e-6, l2
+
+
+def test_c4_ensemble_members_against_oracle():
+    from sddc_amd import cases
+    from sddc_amd.solver import Sd2dEnsemble
+    from oracle.oracle import Oracle
+    print("[gpu-configs] building C4 (64 members)", flush=True)
+    mem = cases.c4_ensemble()
+    ens = Sd2dEnsemble([c for c, _, _ in mem])
+    for i, (c, Q0, _) in enumerate(mem):
+        ens.set_state(i, Q0, [0, 0, 0, 0])
+    ens.step(2)
+    # the first member of each order, against the oracle
+    picks = {}
+    for i, (c, _, d) in enumerate(mem):
+        picks.setdefault(d["N"], i)
+    assert sorted(picks) == [3, 4, 5]
+    for N, i in sorted(picks.items()):
+        c, Q0, d = mem[i]
+        print(f"[gpu-configs] C4 member {i}: {d['nr']}x{d['ntheta']} N={N}", flush=True)
+        o = Oracle(c)
+        o.set_state(Q0, [0, 0, 0, 0])
+        o.step(2)
+        Qo, mo = o.get_state()
+        to, ito = o.get_time()
+        o.close()
+        Qe, me = ens.get_state(i)
+        te, ite = ens.get_time(i)
+        e = per_var_error(Qe - Q0, Qo - Q0)[0]
+        print(f"[gpu-configs] C4 member {i}: increment error {e.max():.2e}", flush=True)
+        assert e.max() <= 1e-11, (i, e)
+        assert np.allclose(me, mo, rtol=1e-13, atol=0), (i, me, mo)
+        assert ite == ito == 2 and abs(te - to) <= 1e-15 * abs(to), (te, to)
+    for i in range(len(mem)):
+        assert np.all(np.isfinite(ens.get_state(i)[0])), i
+    ens.close()
+
+
+def test_c5_adjoint_against_fd():
+    from sddc_amd.adjoint import CylinderObjective, CylinderSpec
+    from sddc_amd.solver import Sd2d
+    print("[gpu-configs] building C5 (128x256, N=4)", flush=True)
+    spec = CylinderSpec(128, 256, 4, mach=1.5, b_angle_deg=10.0)
+    o = CylinderObjective(spec)
+    mach, ang, nsteps, ckpt = spec.mach, spec.b_angle_deg, 20, 5
+    J, gr = o.gradients(mach, ang, nsteps, ckpt)
+    h = 1e-6                                  # Mach, and the angle in radians
+    hd = math.degrees(h)
+    fdm = (o.value(mach + h, ang, nsteps, ckpt) - o.value(mach - h, ang, nsteps, ckpt)) / (2 * h)
+    fda = (o.value(mach, ang + hd, nsteps, ckpt) - o.value(mach, ang - hd, nsteps, ckpt)) / (2 * h)
+    print(f"[gpu-configs] C5 J={J:.10e} dJ/dM {gr['mach']:.10e} fd {fdm:.10e}; "
+          f"dJ/dtheta {gr['angle_rad']:.10e} fd {fda:.10e}", flush=True)
+    assert abs(gr["mach"] - fdm) <= 1e-5 * abs(fdm)
+    assert abs(gr["angle_rad"] - fda) <= 1e-5 * abs(fda)
+    assert gr["av_coefficient"] == 0.0
+    # VJP of a seeded N(0,1) cotangent on the final state plus the drag,
+    # along a seeded direction of the initial state on the 32 innermost
+    # rings (the uniform far field holds exact ties of the running maxima
+    # behind c_h, where the objective is not differentiable)
+    c, Q0 = o.case(mach, ang)
+    rng = np.random.default_rng(20)
+    Cb = np.asfortranarray(rng.standard_normal(Q0.shape))
+    V = rng.standard_normal(Q0.shape) * np.abs(Q0).max(axis=(1, 2, 3))[:, None, None, None]
+    V[..., 32 * spec.ntheta:] = 0.0          # cells are numbered ring by ring
+    jbar = 3.0
+    g = Sd2d(c)
+
+    def F(Q):
+        g.set_state(Q, [0, 0, 0, 0])
+        Jq = g.fwd(nsteps, ckpt)
+        return float(np.sum(Cb * g.get_state()[0])) + jbar * Jq
+
+    g.set_state(Q0, [0, 0, 0, 0])
+    g.fwd(nsteps, ckpt)
+    q0b, _, _ = g.vjp(Cb, jbar)
+    fd = (F(Q0 + h * V) - F(Q0 - h * V)) / (2 * h)
+    ad = float(np.sum(q0b * V))
+    print(f"[gpu-configs] C5 state VJP: adjoint {ad:.10e} fd {fd:.10e} "
+          f"rel {abs(ad - fd) / abs(fd):.2e}", flush=True)
+    assert abs(ad - fd) <= 1e-5 * abs(fd), (ad, fd)
+    g.close()

@@ -38,6 +38,11 @@ def test_stages_equal_steps(name, k_stage):
 
 
 def test_stage_order_enforced():
+    """Stages out of order and, between stage 1 and stage 5 of a step
+    (RK_S5P3 stage 4 writes its output in place), every call that reads or
+    writes the state or evaluates the residual outside the sequence are
+    refused; the running maxima alone can be read; a new state abandons
+    the step."""
     from sddc_amd.solver import Sd2d, Sd2dError
     print("[gpu-rk-stage] order", flush=True)
     case, Q0 = make_case("ogrid4x8_N4")
@@ -52,10 +57,52 @@ def test_stage_order_enforced():
             g.step(1)
         with pytest.raises(Sd2dError):
             g.rk_stage(4)
-        q_mid, _ = g.get_state()
-        assert np.array_equal(q_mid, np.asfortranarray(Q0))   # the step is not complete
-        g.set_state(Q0)                                         # abandons the step
+        g.rk_stage(3)
+        g.rk_stage(4)
+        for call in (g.get_state, g.compresid, lambda: g.write_restart("/nonexistent/x.dat"),
+                     lambda: g.compresid_vjp(np.zeros_like(Q0)), lambda: g.fwd(1, 1)):
+            with pytest.raises(Sd2dError, match="in progress"):
+                call()
+        mx = g.maxima()                                          # allowed mid-step
+        assert np.all(np.isfinite(mx)) and mx[2] > 0.0
+        g.rk_stage(5)
+        q_end, _ = g.get_state()
+        assert not np.array_equal(q_end, np.asfortranarray(Q0))
+        g.set_state(Q0)                                          # a new state ...
+        g.rk_stage(1)
+        g.set_state(Q0)                                          # ... abandons the step
         g.step(1)
         g.rk_stage(1)
     finally:
         g.close()
+
+
+@pytest.mark.parametrize("k_stage", [3, 5])
+def test_diag_c_h_is_the_last_evaluations(k_stage):
+    """sd2d_get_diag's c_h after sd2d_step / sd2d_rk_stage is the one the
+    last stage used: sqrt(eigvmax_g (eigvmax_g - umax_g)) of the maxima
+    before that stage (compflux.f90:6), and equal to the oracle's."""
+    from sddc_amd.solver import Sd2d
+    from oracle.oracle import Oracle
+    print(f"[gpu-rk-stage] diag c_h k_stage={k_stage}", flush=True)
+    case, Q0 = make_case("channel7x6_N5", k_stage=k_stage)
+    g = Sd2d(case)
+    o = Oracle(case)
+    try:
+        g.set_state(Q0, [0.0, 0.0, 0.0, 0.0])
+        o.set_state(Q0, [0.0, 0.0, 0.0, 0.0])
+        assert g.diag()["c_h"] == 0.0
+        for s in range(1, 5):
+            g.rk_stage(s)
+        mx = g.maxima()
+        g.rk_stage(5)
+        ch = np.sqrt(mx[2] * (mx[2] - mx[3]))
+        assert abs(g.diag()["c_h"] - ch) <= 1e-15 * ch, (g.diag()["c_h"], ch)
+        g.step(2)
+        o.step(3)
+        ch_g, ch_o = g.diag()["c_h"], o.diag()["c_h"]
+        assert ch_g > 0.0
+        assert abs(ch_g - ch_o) <= 1e-14 * ch_o, (ch_g, ch_o)
+    finally:
+        g.close()
+        o.close()
