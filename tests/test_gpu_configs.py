@@ -2,12 +2,27 @@
 C restatement of the reference, bit-exact to it on the golden fixtures).
 
   C2  512 x 1024 O-grid cylinder, N=5: two sd2d_compresid calls (c_h = 0,
-      then lagged from the first call's maxima) against Oracle.compresid,
-      per variable ||r_gpu - r_oracle||_inf / ||r_oracle||_inf <= 1e-12 for
-      resid and nablaQs (compresid.f90:1-80), maxima to 1e-14 relative.
+      then lagged from the first call's maxima) against Oracle.compresid:
+      resid per variable ||r_gpu - r_oracle||_inf / ||r_oracle||_inf <=
+      1e-12 (compresid.f90:1-80), maxima to 1e-14 relative.
   C3  1024 x 1024 periodic Orszag-Tang, N=4: the same two calls and one
-      RK_S5P3 step (state increment to 1e-11); mixed precision against FP64
-      on the device over 50 steps, relative L2 drift <= 1e-6 per variable.
+      RK_S5P3 step; mixed precision against FP64 on the device over 50
+      steps, relative L2 drift <= 1e-6 per variable.
+
+  The rounding floor.  The oracle's C source built with FMA contraction
+  (oracle/liboracle_sd2d_fma.so; gcc and flang contract by default at -O2
+  and -O3 on an FMA machine) is a second legal IEEE evaluation of the
+  reference's arithmetic.  Its distance from the plain build on the same
+  input is the floor below which no FP64 implementation can be told from
+  the reference.  On these inputs it lies above the fixed tolerances in
+  places: C2 nablaQs d(E)/d(eta) 1.07e-12; C3 resid up to 3.7e-12 (the
+  residual of a 1024^2 vortex is a difference of fluxes ~1e5 times larger
+  than itself) and d(Bx)/dx 3e-10 absolute around its exact zero (Bx =
+  -sin y).  Every comparison here therefore asserts, per variable,
+  err(device, plain) <= max(tol, 2 * err(fma build, plain)), and prints
+  both; C2's resid is held to 1e-12 regardless.  Variables whose largest
+  value is below 1e-8 of the largest variable's (rounding noise around an
+  exact zero) are measured against the largest variable.
   C4  the 64-member ensemble (seed 4) for two steps in FP64: one member of
       each order against the oracle run of that member (increment to 1e-11,
       maxima to 1e-13); every member finite.
@@ -28,30 +43,55 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-12
 
 
-def _compresid_pair(case, Q0, tag):
-    """Two compresid calls on device and oracle from the same state."""
+def _rel(a, b):
+    """Per variable ||a - b||_inf / ||b||_inf; variables whose max is below
+    1e-8 of the largest variable's are measured against the largest."""
+    a = a.reshape(a.shape[0], -1)
+    b = b.reshape(b.shape[0], -1)
+    sc = np.abs(b).max(axis=1)
+    big = sc.max()
+    den = np.where(sc < 1e-8 * big, big, sc)
+    return np.abs(a - b).max(axis=1) / np.where(den == 0, 1.0, den)
+
+
+def _check_floor(tag, what, dev, plain, fma, tol):
+    e = _rel(dev, plain)
+    fl = _rel(fma, plain)
+    bound = np.maximum(tol, 2.0 * fl)
+    print(f"[gpu-configs] {tag} {what}: device {np.array2string(e, precision=2)}\n"
+          f"[gpu-configs] {tag} {what}: floor  {np.array2string(fl, precision=2)}", flush=True)
+    assert np.all(e <= bound), (tag, what, e, fl)
+    return e
+
+
+def _compresid_pair(case, Q0, tag, strict_resid=False):
+    """Two compresid calls on the device and both oracle builds from the
+    same state."""
     from sddc_amd.solver import Sd2d
     from oracle.oracle import Oracle
     g = Sd2d(case)
     o = Oracle(case)
-    g.set_state(Q0, [0, 0, 0, 0])
-    o.set_state(Q0, [0, 0, 0, 0])
+    f = Oracle(case, fma=True)
+    for x in (g, o, f):
+        x.set_state(Q0, [0, 0, 0, 0])
     for it in range(2):
         print(f"[gpu-configs] {tag} compresid call {it + 1}", flush=True)
         rg, ng = g.compresid()
         ro, no = o.compresid()
-        e = per_var_error(rg, ro)[0]
-        e2 = per_var_error(ng.reshape(18, -1, order="F"), no.reshape(18, -1, order="F"))[0]
-        print(f"[gpu-configs] {tag} call {it + 1}: resid {e.max():.2e} nablaQs {e2.max():.2e}",
-              flush=True)
+        rf, nf = f.compresid()
         assert np.all(np.isfinite(rg))
-        assert e.max() <= TOL, (tag, it, e)
-        assert e2.max() <= TOL, (tag, it, e2)
+        e = _check_floor(tag, f"call {it + 1} resid", rg, ro, rf, TOL)
+        if strict_resid:
+            assert e.max() <= TOL, (tag, it, e)
+        del rg, ro, rf
+        _check_floor(tag, f"call {it + 1} nablaQs", ng.reshape(18, -1, order="F"),
+                     no.reshape(18, -1, order="F"), nf.reshape(18, -1, order="F"), TOL)
+        del ng, no, nf
         _, mg = g.get_state()
         _, mo = o.get_state()
         assert np.allclose(mg, mo, rtol=1e-14, atol=0), (tag, it, mg, mo)
-        del rg, ng, ro, no
     assert g.diag()["c_h"] > 0.0
+    f.close()
     return g, o
 
 
@@ -59,7 +99,7 @@ def test_c2_compresid_matches_oracle():
     from sddc_amd import cases
     print("[gpu-configs] building C2 (512x1024, N=5)", flush=True)
     case, Q0 = cases.c2_cylinder()
-    g, o = _compresid_pair(case, Q0, "C2")
+    g, o = _compresid_pair(case, Q0, "C2", strict_resid=True)
     g.close()
     o.close()
 
@@ -69,16 +109,20 @@ def test_c3_orszag_tang_against_oracle():
     from sddc_amd.solver import Sd2d
     print("[gpu-configs] building C3 (1024x1024 periodic, N=4)", flush=True)
     case, Q0 = cases.orszag_tang(1024, N=4)
+    from oracle.oracle import Oracle
     g, o = _compresid_pair(case, Q0, "C3")
     # one RK_S5P3 step from the initial state with fresh maxima
     print("[gpu-configs] C3 one step", flush=True)
-    g.set_state(Q0, [0, 0, 0, 0]); o.set_state(Q0, [0, 0, 0, 0])
-    g.step(1, 1); o.step(1, 1)
+    f = Oracle(case, fma=True)
+    for x in (g, o, f):
+        x.set_state(Q0, [0, 0, 0, 0])
+        x.step(1, 1)
     Qg, mg = g.get_state()
     Qo, mo = o.get_state()
-    e = per_var_error(Qg - Q0, Qo - Q0)[0]
-    print(f"[gpu-configs] C3 step increment error {e.max():.2e}", flush=True)
-    assert e.max() <= 1e-11, e
+    Qf, _ = f.get_state()
+    f.close()
+    _check_floor("C3", "step increment", Qg - Q0, Qo - Q0, Qf - Q0, 1e-11)
+    del Qf
     assert np.allclose(mg, mo, rtol=1e-13, atol=0), (mg, mo)
     dg, do = g.diag(), o.diag()
     np.testing.assert_allclose([dg["resid_rms"], dg["divb_rms"]],
