@@ -174,6 +174,23 @@ int sd2d_compresid_q(sd2d_ctx *ctx, const double *q,
                      double *resid_out, double *nablaqs_out);
 
 /*
+ * sd2d_compresid without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc), since sd2d_compresid allocates its nablaQs staging on its
+ * first call.  Same contract and bitwise the same results as
+ * sd2d_compresid; the nablaQs staging is a static device buffer of
+ * SD2D_NOALLOC_MAX_DOUBLES doubles (64 MiB) on each GPU, held until the
+ * process ends, so 2*9*N*N*NCELL must not exceed it (SD2D_EINVAL
+ * otherwise; at N = 5 up to 18,641 cells).  Every call on one GPU uses
+ * the same buffer: calls that overlap on one GPU (contexts driven from
+ * several host threads) race.  Profiling (sd2d_set_profiling) must be off.
+ */
+#define SD2D_NOALLOC_MAX_DOUBLES (8 * 1024 * 1024)
+int sd2d_compresid_noalloc(sd2d_ctx *ctx, double *resid_out,
+                           double *nablaqs_out);
+
+/*
  * Device-resident time stepping: `nsteps` calls of RK_S5P3
  * (iterations.f90:158-213) or FiveStageRk (iterations.f90:88-156) as chosen
  * by params.k_stage, including the ctime bookkeeping.  When norm_every > 0,

@@ -17,6 +17,8 @@ SD2D_OK, SD2D_EINVAL, SD2D_EHIP, SD2D_ENOMEM, SD2D_ENONFINITE, \
     SD2D_EUNSUPPORTED = range(6)
 SD2D_FP64, SD2D_MIXED = 0, 1
 SD2D_TILE_DEFAULT, SD2D_TILE_STRIPS, SD2D_TILE_BLOCKS = 0, 1, 2
+# doubles of the static scratch of sd2d_compresid_noalloc (64 MiB)
+SD2D_NOALLOC_MAX_DOUBLES = 8 * 1024 * 1024
 
 _pi = C.POINTER(C.c_int32)
 _pd = C.POINTER(C.c_double)

@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -88,6 +89,29 @@ struct sd2d_ctx {
 };
 
 static std::string g_err = "no error";
+
+// The nablaQs staging of sd2d_compresid_noalloc: a file-scope device array,
+// so that entry point allocates nothing.  Each GPU holds its own copy for
+// the life of the process; its address on a device comes from
+// hipGetSymbolAddress with that device current (in host code the array's
+// name is only a host-side stand-in and must never be passed to a kernel
+// or a copy).  Calls that overlap on one GPU share it and race.
+static __device__ __align__(256) double scratch[SD2D_NOALLOC_MAX_DOUBLES];
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static double *g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+static hipError_t g_scratch_err[NOALLOC_MAX_DEVICES] = {};
+static std::mutex g_scratch_mu;
+
+// Look up (once per device, the device current) the address of `scratch`.
+static void scratch_init(int dev) {
+  if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[dev]) return;
+  void *ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  g_scratch_err[dev] = e;
+  if (e == hipSuccess) g_scratch_ptr[dev] = static_cast<double *>(ptr);
+}
 
 // vismode 0 is refused: the reference then forms the Powell source and the
 // divB norm from a nablaQs it never computes (compflux.f90:13-15 against
@@ -448,6 +472,7 @@ static int ctx_upload(sd2d_ctx *c, const HostTables &tb, int nwall, int nmem,
     c->sfam = cross[0] < cross[1] ? 0 : 1;
   }
   CK(hipGetDevice(&c->device));
+  scratch_init(c->device);
   CK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   const size_t qn = (size_t)nt * 9 * N2 * CPW;
   for (int i = 0; i < 3; ++i) CK(hipMalloc(&c->q[i], qn * sizeof(double)));
@@ -695,6 +720,53 @@ int sd2d_compresid_q(sd2d_ctx *c, const double *q, double *resid_out, double *na
   int r = sd2d_set_state(c, q, nullptr);
   if (r) return r;
   return sd2d_compresid(c, resid_out, nablaqs_out);
+}
+
+// sd2d_compresid without any allocation: the same kernels and per-call
+// contract, with the Fortran-layout nablaQs staged in `scratch` (through the
+// address looked up for this context's device) instead of a buffer the
+// library allocates on the first call.  The COMPAT stage kernel writes every
+// element of that range (the xi-line threads store, the eta-line threads add
+// after a barrier), so nothing from an earlier call is read.
+int sd2d_compresid_noalloc(sd2d_ctx *c, double *resid_out, double *nablaqs_out) {
+  if (!c) return fail(nullptr, SD2D_EINVAL, "ctx is NULL");
+  SD2D_NOT_MID_STEP(c, "sd2d_compresid_noalloc");
+  const size_t n = (size_t)c->ncell * 9 * c->N * c->N;
+  if (2 * n > (size_t)SD2D_NOALLOC_MAX_DOUBLES)
+    return fail(c, SD2D_EINVAL,
+                "sd2d_compresid_noalloc: nablaQs needs " + std::to_string(2 * n) +
+                    " doubles of scratch; the static buffer holds " +
+                    std::to_string((size_t)SD2D_NOALLOC_MAX_DOUBLES) + " (use sd2d_compresid)");
+  if (c->device < 0 || c->device >= NOALLOC_MAX_DEVICES || !g_scratch_ptr[c->device])
+    return fail(c, SD2D_EHIP,
+                std::string("sd2d_compresid_noalloc: no address of the static scratch on this device: ") +
+                    (c->device >= 0 && c->device < NOALLOC_MAX_DEVICES
+                         ? hipGetErrorString(g_scratch_err[c->device])
+                         : "device index out of range"));
+  if (c->prof)
+    return fail(c, SD2D_EINVAL,
+                "sd2d_compresid_noalloc: profiling creates HIP events; switch it off first");
+  double *nab = g_scratch_ptr[c->device];
+  StageArgs a = base_args(c);
+  a.qin = c->q[c->cur];
+  a.resid_f = c->stagef;
+  a.nabla_f = nab;
+  int mode = M_COMPAT | (c->p.artiv == 1 && c->p.vismode == 1 ? M_AV : 0);
+  if (c->vis) {
+    HIPCHK(c, launch_vis(c, a.qin));
+    a.visres = c->visres;
+    mode |= M_VIS;
+  }
+  HIPCHK(c, launch_stage_raw(c, a, mode));
+  c->have_av = (mode & M_AV) != 0;
+  const int r = finalize_stage(c);
+  if (r) return r;
+  if (resid_out)
+    HIPCHK(c, hipMemcpyAsync(resid_out, c->stagef, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (nablaqs_out)
+    HIPCHK(c, hipMemcpyAsync(nablaqs_out, nab, 2 * n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return SD2D_OK;
 }
 
 // One RK step, or (only > 0) only stage `only` of it: the sequence is walked

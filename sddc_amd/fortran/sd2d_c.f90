@@ -119,6 +119,16 @@ module sd2d_c
        real(c_double), intent(out), optional :: nablaqs(*)
      end function sd2d_compresid_q
 
+     ! no counterpart in the reference: sd2d_compresid without any device
+     ! allocation, for hosts that cannot allocate (graph capture, serving loops)
+     integer(c_int) function sd2d_compresid_noalloc(ctx, resid, nablaqs) &
+          bind(C, name="sd2d_compresid_noalloc")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out), optional :: resid(*)
+       real(c_double), intent(out), optional :: nablaqs(*)
+     end function sd2d_compresid_noalloc
+
      integer(c_int) function sd2d_step(ctx, nsteps, norm_every) bind(C, name="sd2d_step")
        import :: c_int, c_ptr, c_int32_t
        type(c_ptr), value :: ctx

@@ -25,7 +25,7 @@ _lib = None
 
 EXPORTS = ("sd2d_create", "sd2d_destroy", "sd2d_last_error", "sd2d_set_state",
            "sd2d_get_state", "sd2d_set_time", "sd2d_compresid",
-           "sd2d_compresid_q", "sd2d_step", "sd2d_rk_stage", "sd2d_get_diag", "sd2d_get_av",
+           "sd2d_compresid_q", "sd2d_compresid_noalloc", "sd2d_step", "sd2d_rk_stage", "sd2d_get_diag", "sd2d_get_av",
            "sd2d_write_restart", "sd2d_read_restart",
            "sd2d_set_profiling", "sd2d_get_profile",
            "sd2d_ens_create", "sd2d_ens_destroy", "sd2d_ens_last_error",
@@ -63,6 +63,7 @@ def load_library(path: str = LIBPATH):
     L.sd2d_set_time.argtypes = [vp, C.c_double, C.c_int32]
     L.sd2d_compresid.argtypes = [vp, vp, vp]
     L.sd2d_compresid_q.argtypes = [vp, vp, vp, vp]
+    L.sd2d_compresid_noalloc.argtypes = [vp, vp, vp]
     L.sd2d_step.argtypes = [vp, C.c_int32, C.c_int32]
     L.sd2d_rk_stage.argtypes = [vp, C.c_int32]
     L.sd2d_get_diag.argtypes = [vp, C.POINTER(abi.Sd2dDiag)]

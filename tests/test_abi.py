@@ -274,3 +274,14 @@ def test_fortran_module_declares_every_entry_point():
     src = open(os.path.join(os.path.dirname(HERE), "sddc_amd", "fortran", "sd2d_c.f90")).read()
     bound = sorted(set(re.findall(r'bind\(C,\s*name="(sd2d_\w+)"\)', src)))
     assert bound == declared_functions()
+
+
+def test_noalloc_limit_matches_header():
+    """The scratch limit of sd2d_compresid_noalloc in the ctypes mirror is
+    the header's SD2D_NOALLOC_MAX_DOUBLES."""
+    m = re.search(r"#define\s+SD2D_NOALLOC_MAX_DOUBLES\s+\(([\d\s\*]+)\)", open(HEADER).read())
+    assert m, "SD2D_NOALLOC_MAX_DOUBLES not defined in include/sd2d.h"
+    val = 1
+    for f in m.group(1).split("*"):
+        val *= int(f)
+    assert val == abi.SD2D_NOALLOC_MAX_DOUBLES
