@@ -230,6 +230,23 @@ int sd2d_get_av(sd2d_ctx *ctx, double *sm_ind, double *muav,
                 double *epsilon0);
 
 /*
+ * Nodal values of the Tecplot output (tecplotter.f90:17-60) from the device
+ * state: plotq(9,nnode) = rho, u, v, w, p, Bx, By, Bz, psi at every node of
+ * a plot mesh refined NRE x NRE per cell, and plotav(2,nnode) = muAV,
+ * etaAV, each the average over the cells that hold the node, in the
+ * reference's arithmetic (bitwise its values for the same state).  The plot
+ * mesh comes from the host as CSR: the occurrences of node i are
+ * occ[xadj[i] .. xadj[i+1]-1], ascending in cell, each coded
+ * cell*(nre+1)^2 + ii + (nre+1)*jj (0-based mesh cell and node position
+ * (ii/nre, jj/nre) in the cell); muav(2,NCELL) is the per-cell muAV of
+ * sd2d_get_av (plotav may be NULL, then muav too).  The mesh numbering of
+ * tecplotter2dsetup.f90 and the #!TDV111 file are the host's part
+ * (sddc_amd/tecplot.py).
+ */
+int sd2d_plot_nodes(sd2d_ctx *ctx, int32_t nre, int32_t nnode, const int32_t *xadj,
+                    const int32_t *occ, const double *muav, double *plotq, double *plotav);
+
+/*
  * Restart files in the reference's format (WRITE_ALL_DATA_BINARY_MPI,
  * write_data.f90:44-74; READ_ALL_DATA_BINARY, read_data_restart.f90:35-67):
  * int32 iter, float64 ctime, then one record per cell: int32 global cell
@@ -247,6 +264,22 @@ int sd2d_get_av(sd2d_ctx *ctx, double *sm_ind, double *muav,
  */
 int sd2d_write_restart(sd2d_ctx *ctx, const char *path, const int32_t *icg);
 int sd2d_read_restart(sd2d_ctx *ctx, const char *path);
+
+/*
+ * Checkpoints that resume bit for bit, no counterpart in the reference.
+ * The reference's restart file holds only iter, ctime and Q, and a
+ * restarted run begins with the running maxima at 0 (initsetup.f90:154-167),
+ * so its first c_h differs from the continuous run's.  sd2d_write_checkpoint
+ * writes the restart file exactly as sd2d_write_restart and, beside it, the
+ * sidecar <path>.max: 8 bytes "SD2DMAX1", int32 iter, float64 ctime (those
+ * of the restart file), float64 eigvmax, umax, eigvmax_g, umax_g (native
+ * byte order).  sd2d_read_checkpoint reads both and restores the maxima, so
+ * stepping on gives bitwise the continuous run; a missing sidecar or one of
+ * another step is SD2D_EINVAL.  The restart file alone still reads with
+ * sd2d_read_restart (and by the reference program).
+ */
+int sd2d_write_checkpoint(sd2d_ctx *ctx, const char *path, const int32_t *icg);
+int sd2d_read_checkpoint(sd2d_ctx *ctx, const char *path);
 
 /*
  * Measurement aid, no counterpart in the reference.  While profiling is on,
@@ -304,6 +337,11 @@ int sd2d_ens_set_state(sd2d_ens *ens, int32_t member, const double *q, const sd2
 int sd2d_ens_get_state(sd2d_ens *ens, int32_t member, double *q, sd2d_maxima *maxima);
 int sd2d_ens_set_time(sd2d_ens *ens, int32_t member, double ctime, int32_t iter);
 int sd2d_ens_get_time(sd2d_ens *ens, int32_t member, double *ctime, int32_t *iter);
+/* A member's checkpoint (restart file of its state, iter and ctime, plus
+ * the running-maxima sidecar), with the format and contract of
+ * sd2d_write_checkpoint / sd2d_read_checkpoint. */
+int sd2d_ens_write_checkpoint(sd2d_ens *ens, int32_t member, const char *path, const int32_t *icg);
+int sd2d_ens_read_checkpoint(sd2d_ens *ens, int32_t member, const char *path);
 int sd2d_ens_step(sd2d_ens *ens, int32_t nsteps);
 int sd2d_ens_tune(sd2d_ens *ens, int32_t trial_steps, int32_t *shapes);
 /* The finiteness check of sd2d_set_check_finite for every member: with

@@ -258,6 +258,73 @@ __global__ void k_norm_reduce(const double *__restrict__ nb, int ntile,
   if (t == 0) { out[0] = s[0][0]; out[1] = s[1][0]; }
 }
 
+// Nodal values of the Tecplot output (tecplotter.f90:17-60), one thread
+// per plot node: for every (cell, ii, jj) occurrence of the node, in
+// ascending cell order, the cell's solution polynomial at the node,
+// sum_js sum_is (Q(is,js) * h(ii,is)) * h(jj,js) from 0; the occurrences
+// summed from 0, divided by their count, converted to rho, u, v, w, p, Bx,
+// By, Bz, psi; muAV(1:2) averaged likewise.  Plain IEEE double in the
+// reference's order (no contraction to FMA), so the values are bitwise the
+// reference program's for the same state.
+template <int N>
+__global__ void k_plot_nodes(const double *__restrict__ q, const int *__restrict__ c2s, int nnode,
+                             const int *__restrict__ xadj, const int *__restrict__ occ, int np1,
+                             const double *__restrict__ h, const double *__restrict__ muav,
+                             double *__restrict__ plotq, double *__restrict__ plotav) {
+#pragma clang fp contract(off)
+  constexpr int T = Geo<N>::CPW, N2 = N * N, QT = Geo<N>::QT;
+  const int nd = blockIdx.x * blockDim.x + threadIdx.x;
+  if (nd >= nnode) return;
+  double acc[9];
+#pragma unroll
+  for (int v = 0; v < 9; ++v) acc[v] = 0.0;
+  double a0 = 0.0, a1 = 0.0;
+  const int k0 = xadj[nd], k1 = xadj[nd + 1];
+  for (int k = k0; k < k1; ++k) {
+    const int code = occ[k];
+    const int cell = code / (np1 * np1), loc = code % (np1 * np1);
+    const int ii = loc % np1, jj = loc / np1;
+    const int sl = c2s[cell];
+    const double *b = q + (size_t)(sl / T) * QT + sl % T;
+    double qs[9];
+#pragma unroll
+    for (int v = 0; v < 9; ++v) qs[v] = 0.0;
+    for (int js = 0; js < N; ++js)
+      for (int is = 0; is < N; ++is) {
+        const double hi = h[ii * N + is], hj = h[jj * N + js];
+#pragma unroll
+        for (int v = 0; v < 9; ++v) qs[v] = qs[v] + (b[(v * N2 + is + N * js) * T] * hi) * hj;
+      }
+#pragma unroll
+    for (int v = 0; v < 9; ++v) acc[v] = acc[v] + qs[v];
+    if (muav) {
+      a0 = a0 + muav[2 * (size_t)cell];
+      a1 = a1 + muav[2 * (size_t)cell + 1];
+    }
+  }
+  const double f = (double)(k1 - k0);
+  double Qs[9];
+#pragma unroll
+  for (int v = 0; v < 9; ++v) Qs[v] = acc[v] / f;
+  double *o = plotq + 9 * (size_t)nd;
+  const double u = Qs[1] / Qs[0], vv = Qs[2] / Qs[0], w = Qs[3] / Qs[0];
+  o[0] = Qs[0];
+  o[1] = u;
+  o[2] = vv;
+  o[3] = w;
+  o[5] = Qs[5];
+  o[6] = Qs[6];
+  o[7] = Qs[7];
+  o[8] = Qs[8];
+  const double nu = u * u + vv * vv + w * w;
+  const double nb = Qs[5] * Qs[5] + Qs[6] * Qs[6] + Qs[7] * Qs[7];
+  o[4] = (Qs[4] - 0.5 * Qs[0] * nu - 0.5 * nb) * (GAM - 1);
+  if (plotav) {
+    plotav[2 * (size_t)nd] = a0 / f;
+    plotav[2 * (size_t)nd + 1] = a1 / f;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // stage launch
 // ---------------------------------------------------------------------------
@@ -857,61 +924,140 @@ int sd2d_step(sd2d_ctx *c, int32_t nsteps, int32_t norm_every) {
   return SD2D_OK;
 }
 
-int sd2d_write_restart(sd2d_ctx *c, const char *path, const int32_t *icg) {
-  if (!c || !path) return fail(c, SD2D_EINVAL, "NULL argument");
-  SD2D_NOT_MID_STEP(c, "sd2d_write_restart");
-  const size_t rec = (size_t)9 * c->N * c->N;
-  std::vector<double> q(rec * c->ncell);
-  int r = sd2d_get_state(c, q.data(), nullptr);
-  if (r) return r;
+// The reference's restart stream (write_data.f90:44-74): int32 iter,
+// float64 ctime, then per cell int32 ICG and float64 Q(9,N,N); q holds
+// Q(9,N,N,NCELL) in Fortran layout.
+static bool write_restart_file(const char *path, int32_t iter, double ctime, int N, int ncell,
+                               const double *q, const int32_t *icg) {
+  const size_t rec = (size_t)9 * N * N;
   FILE *fh = std::fopen(path, "wb");
-  if (!fh) return fail(c, SD2D_EINVAL, std::string("cannot open ") + path);
-  bool ok = std::fwrite(&c->iter, sizeof(int32_t), 1, fh) == 1 &&
-            std::fwrite(&c->ctime, sizeof(double), 1, fh) == 1;
-  for (int ic = 0; ok && ic < c->ncell; ++ic) {
+  if (!fh) return false;
+  bool ok = std::fwrite(&iter, sizeof(int32_t), 1, fh) == 1 &&
+            std::fwrite(&ctime, sizeof(double), 1, fh) == 1;
+  for (int ic = 0; ok && ic < ncell; ++ic) {
     const int32_t g = icg ? icg[ic] : ic + 1;
     ok = std::fwrite(&g, sizeof(int32_t), 1, fh) == 1 &&
-         std::fwrite(q.data() + rec * ic, sizeof(double), rec, fh) == rec;
+         std::fwrite(q + rec * ic, sizeof(double), rec, fh) == rec;
   }
-  ok = (std::fclose(fh) == 0) && ok;
-  if (!ok) return fail(c, SD2D_EINVAL, std::string("write failed: ") + path);
-  return SD2D_OK;
+  return (std::fclose(fh) == 0) && ok;
 }
 
-int sd2d_read_restart(sd2d_ctx *c, const char *path) {
-  if (!c || !path) return fail(c, SD2D_EINVAL, "NULL argument");
-  const size_t rec = (size_t)9 * c->N * c->N;
+// READ_ALL_DATA_BINARY (read_data_restart.f90:35-67) for one rank holding
+// the whole mesh: every record's cell goes to local cell ICG-1.  Returns an
+// empty string or what is wrong.
+static std::string read_restart_file(const char *path, int N, int ncell, int32_t &iter,
+                                     double &ctime, std::vector<double> &q) {
+  const size_t rec = (size_t)9 * N * N;
   FILE *fh = std::fopen(path, "rb");
-  if (!fh) return fail(c, SD2D_EINVAL, std::string("cannot open ") + path);
+  if (!fh) return std::string("cannot open ") + path;
   std::fseek(fh, 0, SEEK_END);
   const long size = std::ftell(fh);
   std::fseek(fh, 0, SEEK_SET);
   const long want = (long)(sizeof(int32_t) + sizeof(double) +
-                           (sizeof(int32_t) + rec * sizeof(double)) * (size_t)c->ncell);
+                           (sizeof(int32_t) + rec * sizeof(double)) * (size_t)ncell);
   if (size != want) {
     std::fclose(fh);
-    return fail(c, SD2D_EINVAL, "restart file size does not match NCELL and N");
+    return "restart file size does not match NCELL and N";
   }
-  int32_t it = 0;
-  double t = 0.0;
-  std::vector<double> q(rec * c->ncell, 0.0);
-  std::vector<char> seen(c->ncell, 0);
-  bool ok = std::fread(&it, sizeof(int32_t), 1, fh) == 1 && std::fread(&t, sizeof(double), 1, fh) == 1;
-  for (int i = 0; ok && i < c->ncell; ++i) {
+  q.assign(rec * ncell, 0.0);
+  std::vector<char> seen(ncell, 0);
+  bool ok = std::fread(&iter, sizeof(int32_t), 1, fh) == 1 &&
+            std::fread(&ctime, sizeof(double), 1, fh) == 1;
+  for (int i = 0; ok && i < ncell; ++i) {
     int32_t g = 0;
     ok = std::fread(&g, sizeof(int32_t), 1, fh) == 1;
     if (!ok) break;
-    if (g < 1 || g > c->ncell || seen[g - 1]) {
+    if (g < 1 || g > ncell || seen[g - 1]) {
       std::fclose(fh);
-      return fail(c, SD2D_EINVAL, "restart record with a cell number outside 1..NCELL or repeated");
+      return "restart record with a cell number outside 1..NCELL or repeated";
     }
     seen[g - 1] = 1;
     ok = std::fread(q.data() + rec * (g - 1), sizeof(double), rec, fh) == rec;
   }
   std::fclose(fh);
-  if (!ok) return fail(c, SD2D_EINVAL, std::string("read failed: ") + path);
+  if (!ok) return std::string("read failed: ") + path;
+  return std::string();
+}
+
+// The running-maxima sidecar of a checkpoint, <restart path>.max: 8 bytes
+// "SD2DMAX1", int32 iter, float64 ctime (those of the restart file beside
+// it), float64 eigvmax, umax, eigvmax_g, umax_g; native byte order.
+static std::string max_path(const char *path) { return std::string(path) + ".max"; }
+
+static bool write_max_file(const char *path, int32_t iter, double ctime, const sd2d_maxima &m) {
+  FILE *fh = std::fopen(max_path(path).c_str(), "wb");
+  if (!fh) return false;
+  const double v[4] = {m.eigvmax, m.umax, m.eigvmax_g, m.umax_g};
+  bool ok = std::fwrite("SD2DMAX1", 1, 8, fh) == 8 && std::fwrite(&iter, sizeof(int32_t), 1, fh) == 1 &&
+            std::fwrite(&ctime, sizeof(double), 1, fh) == 1 && std::fwrite(v, sizeof(double), 4, fh) == 4;
+  return (std::fclose(fh) == 0) && ok;
+}
+
+static std::string read_max_file(const char *path, int32_t iter, double ctime, sd2d_maxima &m) {
+  const std::string mp = max_path(path);
+  FILE *fh = std::fopen(mp.c_str(), "rb");
+  if (!fh) return "cannot open the maxima sidecar " + mp;
+  char magic[8];
+  int32_t it = 0;
+  double t = 0.0, v[4];
+  const bool ok = std::fread(magic, 1, 8, fh) == 8 && std::fread(&it, sizeof(int32_t), 1, fh) == 1 &&
+                  std::fread(&t, sizeof(double), 1, fh) == 1 && std::fread(v, sizeof(double), 4, fh) == 4 &&
+                  std::fgetc(fh) == EOF;
+  std::fclose(fh);
+  if (!ok || std::memcmp(magic, "SD2DMAX1", 8) != 0) return "not a maxima sidecar: " + mp;
+  if (it != iter || std::memcmp(&t, &ctime, sizeof(double)) != 0)
+    return "the maxima sidecar " + mp + " belongs to another step than the restart file";
+  m.eigvmax = v[0]; m.umax = v[1]; m.eigvmax_g = v[2]; m.umax_g = v[3];
+  return std::string();
+}
+
+int sd2d_write_restart(sd2d_ctx *c, const char *path, const int32_t *icg) {
+  if (!c || !path) return fail(c, SD2D_EINVAL, "NULL argument");
+  SD2D_NOT_MID_STEP(c, "sd2d_write_restart");
+  std::vector<double> q((size_t)9 * c->N * c->N * c->ncell);
+  int r = sd2d_get_state(c, q.data(), nullptr);
+  if (r) return r;
+  if (!write_restart_file(path, c->iter, c->ctime, c->N, c->ncell, q.data(), icg))
+    return fail(c, SD2D_EINVAL, std::string("write failed: ") + path);
+  return SD2D_OK;
+}
+
+int sd2d_read_restart(sd2d_ctx *c, const char *path) {
+  if (!c || !path) return fail(c, SD2D_EINVAL, "NULL argument");
+  int32_t it = 0;
+  double t = 0.0;
+  std::vector<double> q;
+  const std::string e = read_restart_file(path, c->N, c->ncell, it, t, q);
+  if (!e.empty()) return fail(c, SD2D_EINVAL, e);
   const sd2d_maxima zero{0.0, 0.0, 0.0, 0.0};
   int r = sd2d_set_state(c, q.data(), &zero);
+  if (r) return r;
+  return sd2d_set_time(c, t, it);
+}
+
+int sd2d_write_checkpoint(sd2d_ctx *c, const char *path, const int32_t *icg) {
+  if (!c || !path) return fail(c, SD2D_EINVAL, "NULL argument");
+  SD2D_NOT_MID_STEP(c, "sd2d_write_checkpoint");
+  std::vector<double> q((size_t)9 * c->N * c->N * c->ncell);
+  sd2d_maxima m{};
+  int r = sd2d_get_state(c, q.data(), &m);
+  if (r) return r;
+  if (!write_restart_file(path, c->iter, c->ctime, c->N, c->ncell, q.data(), icg) ||
+      !write_max_file(path, c->iter, c->ctime, m))
+    return fail(c, SD2D_EINVAL, std::string("write failed: ") + path);
+  return SD2D_OK;
+}
+
+int sd2d_read_checkpoint(sd2d_ctx *c, const char *path) {
+  if (!c || !path) return fail(c, SD2D_EINVAL, "NULL argument");
+  int32_t it = 0;
+  double t = 0.0;
+  std::vector<double> q;
+  std::string e = read_restart_file(path, c->N, c->ncell, it, t, q);
+  sd2d_maxima m{};
+  if (e.empty()) e = read_max_file(path, it, t, m);
+  if (!e.empty()) return fail(c, SD2D_EINVAL, e);
+  int r = sd2d_set_state(c, q.data(), &m);
   if (r) return r;
   return sd2d_set_time(c, t, it);
 }
@@ -959,6 +1105,87 @@ int sd2d_get_diag(sd2d_ctx *c, sd2d_diag *d) {
   d->ctime = c->ctime;
   d->iter = c->iter;
   d->norms_valid = c->norms_valid;
+  return SD2D_OK;
+}
+
+int sd2d_plot_nodes(sd2d_ctx *c, int32_t nre, int32_t nnode, const int32_t *xadj,
+                    const int32_t *occ, const double *muav, double *plotq, double *plotav) {
+  if (!c || !xadj || !occ || !plotq) return fail(c, SD2D_EINVAL, "NULL argument");
+  SD2D_NOT_MID_STEP(c, "sd2d_plot_nodes");
+  if (nre < 1 || nnode < 1) return fail(c, SD2D_EINVAL, "nre and nnode must be positive");
+  if (plotav && !muav) return fail(c, SD2D_EINVAL, "plotav needs muav");
+  const long long np1 = nre + 1, per = np1 * np1;
+  if ((long long)c->ncell * per > 0x7fffffffLL)
+    return fail(c, SD2D_EINVAL, "NCELL*(nre+1)^2 exceeds 32-bit occurrence codes");
+  // every index the kernel will use, checked here
+  if (xadj[0] != 0) return fail(c, SD2D_EINVAL, "xadj[0] must be 0");
+  for (int i = 0; i < nnode; ++i)
+    if (xadj[i + 1] <= xadj[i]) return fail(c, SD2D_EINVAL, "xadj must increase strictly");
+  const int nocc = xadj[nnode];
+  for (int k = 0; k < nocc; ++k)
+    if (occ[k] < 0 || (long long)occ[k] >= (long long)c->ncell * per)
+      return fail(c, SD2D_EINVAL, "occurrence code outside NCELL*(nre+1)^2");
+  // Lagrange values of the solution-point basis at the plot positions
+  // (ii)/nre, the product formula of tecplotter.f90:62-80
+  const int N = c->N;
+  std::vector<double> h((size_t)np1 * N);
+  for (int k = 0; k <= nre; ++k) {
+    const double x = (double)k / (double)nre;
+    for (int i = 0; i < N; ++i) {
+      double hn = 1.0, hd = 1.0;
+      for (int s2 = 0; s2 < N; ++s2)
+        if (s2 != i) {
+          hn = hn * (x - c->xs[s2]);
+          hd = hd * (c->xs[i] - c->xs[s2]);
+        }
+      h[(size_t)k * N + i] = hn / hd;
+    }
+  }
+  int *dx = nullptr, *docc = nullptr;
+  double *dh = nullptr, *dmu = nullptr, *dq = nullptr, *dav = nullptr;
+  auto release = [&]() {
+    void *ptrs[] = {dx, docc, dh, dmu, dq, dav};
+    for (void *b : ptrs) if (b) (void)hipFree(b);
+  };
+#define PLOTCHK(call)                                                              \
+  do {                                                                             \
+    hipError_t e_ = (call);                                                        \
+    if (e_ != hipSuccess) {                                                        \
+      release();                                                                   \
+      return fail(c, SD2D_EHIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    }                                                                              \
+  } while (0)
+  PLOTCHK(hipMalloc(&dx, ((size_t)nnode + 1) * sizeof(int)));
+  PLOTCHK(hipMalloc(&docc, (size_t)nocc * sizeof(int)));
+  PLOTCHK(hipMalloc(&dh, h.size() * sizeof(double)));
+  PLOTCHK(hipMalloc(&dq, (size_t)9 * nnode * sizeof(double)));
+  if (plotav) {
+    PLOTCHK(hipMalloc(&dmu, (size_t)2 * c->ncell * sizeof(double)));
+    PLOTCHK(hipMalloc(&dav, (size_t)2 * nnode * sizeof(double)));
+    PLOTCHK(hipMemcpyAsync(dmu, muav, (size_t)2 * c->ncell * sizeof(double), hipMemcpyHostToDevice,
+                           c->stream));
+  }
+  PLOTCHK(hipMemcpyAsync(dx, xadj, ((size_t)nnode + 1) * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  PLOTCHK(hipMemcpyAsync(docc, occ, (size_t)nocc * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  PLOTCHK(hipMemcpyAsync(dh, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  const dim3 grid((nnode + 255) / 256), block(256);
+  const double *qd = c->q[c->cur];
+  switch (N) {
+    case 3: hipLaunchKernelGGL(k_plot_nodes<3>, grid, block, 0, c->stream, qd, c->c2s, nnode, dx, docc,
+                               (int)np1, dh, dmu, dq, dav); break;
+    case 4: hipLaunchKernelGGL(k_plot_nodes<4>, grid, block, 0, c->stream, qd, c->c2s, nnode, dx, docc,
+                               (int)np1, dh, dmu, dq, dav); break;
+    default: hipLaunchKernelGGL(k_plot_nodes<5>, grid, block, 0, c->stream, qd, c->c2s, nnode, dx, docc,
+                                (int)np1, dh, dmu, dq, dav); break;
+  }
+  PLOTCHK(hipGetLastError());
+  PLOTCHK(hipMemcpyAsync(plotq, dq, (size_t)9 * nnode * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (plotav)
+    PLOTCHK(hipMemcpyAsync(plotav, dav, (size_t)2 * nnode * sizeof(double), hipMemcpyDeviceToHost,
+                           c->stream));
+  PLOTCHK(hipStreamSynchronize(c->stream));
+#undef PLOTCHK
+  release();
   return SD2D_OK;
 }
 

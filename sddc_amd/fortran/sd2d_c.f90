@@ -58,11 +58,15 @@ module sd2d_c
 
   public :: sd2d_create, sd2d_destroy, sd2d_last_error, sd2d_set_state, &
        sd2d_get_state, sd2d_set_time, sd2d_compresid, sd2d_compresid_q, &
-       sd2d_step, sd2d_rk_stage, sd2d_get_diag, sd2d_get_av, sd2d_write_restart, &
-       sd2d_read_restart, sd2d_set_profiling, sd2d_get_profile, sd2d_error_message, &
-       sd2d_ens_create, sd2d_ens_destroy, sd2d_ens_last_error, sd2d_ens_set_state, &
-       sd2d_ens_get_state, sd2d_ens_set_time, sd2d_ens_get_time, sd2d_ens_step, &
-       sd2d_ens_tune, sd2d_fwd, sd2d_vjp, sd2d_compresid_vjp
+       sd2d_compresid_noalloc, sd2d_step, sd2d_rk_stage, sd2d_get_diag, &
+       sd2d_get_av, sd2d_plot_nodes, sd2d_write_restart, sd2d_read_restart, sd2d_write_checkpoint, &
+       sd2d_read_checkpoint, sd2d_set_profiling, sd2d_get_profile, &
+       sd2d_set_check_finite, sd2d_ens_create, sd2d_ens_destroy, &
+       sd2d_ens_last_error, sd2d_ens_set_state, sd2d_ens_get_state, &
+       sd2d_ens_set_time, sd2d_ens_get_time, sd2d_ens_write_checkpoint, &
+       sd2d_ens_read_checkpoint, sd2d_ens_step, sd2d_ens_tune, &
+       sd2d_ens_set_check_finite, sd2d_ens_nonfinite, sd2d_fwd, sd2d_vjp, &
+       sd2d_compresid_vjp, sd2d_error_message
 
   interface
      integer(c_int) function sd2d_create(mesh, params, ctx) bind(C, name="sd2d_create")
@@ -155,6 +159,18 @@ module sd2d_c
        real(c_double), intent(out), optional :: epsilon0
      end function sd2d_get_av
 
+     ! nodal values of the Tecplot output (plot mesh from the host, CSR)
+     integer(c_int) function sd2d_plot_nodes(ctx, nre, nnode, xadj, occ, muav, plotq, plotav) &
+          bind(C, name="sd2d_plot_nodes")
+       import :: c_int, c_ptr, c_int32_t, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: nre, nnode
+       integer(c_int32_t), intent(in) :: xadj(*), occ(*)
+       real(c_double), intent(in), optional :: muav(*)
+       real(c_double), intent(out) :: plotq(*)
+       real(c_double), intent(out), optional :: plotav(*)
+     end function sd2d_plot_nodes
+
      ! path: a NUL-terminated string, e.g. trim(name)//c_null_char
      integer(c_int) function sd2d_write_restart(ctx, path, icg) bind(C, name="sd2d_write_restart")
        import :: c_int, c_ptr, c_char, c_int32_t
@@ -168,6 +184,21 @@ module sd2d_c
        type(c_ptr), value :: ctx
        character(kind=c_char), intent(in) :: path(*)
      end function sd2d_read_restart
+
+     ! no counterpart in the reference: restart file plus running-maxima sidecar
+     integer(c_int) function sd2d_write_checkpoint(ctx, path, icg) &
+          bind(C, name="sd2d_write_checkpoint")
+       import :: c_int, c_ptr, c_char, c_int32_t
+       type(c_ptr), value :: ctx
+       character(kind=c_char), intent(in) :: path(*)
+       integer(c_int32_t), intent(in), optional :: icg(*)
+     end function sd2d_write_checkpoint
+
+     integer(c_int) function sd2d_read_checkpoint(ctx, path) bind(C, name="sd2d_read_checkpoint")
+       import :: c_int, c_ptr, c_char
+       type(c_ptr), value :: ctx
+       character(kind=c_char), intent(in) :: path(*)
+     end function sd2d_read_checkpoint
 
      integer(c_int) function sd2d_set_profiling(ctx, on) bind(C, name="sd2d_set_profiling")
        import :: c_int, c_ptr, c_int32_t
@@ -246,6 +277,23 @@ module sd2d_c
        real(c_double), intent(out), optional :: ctime
        integer(c_int32_t), intent(out), optional :: iter
      end function sd2d_ens_get_time
+
+     integer(c_int) function sd2d_ens_write_checkpoint(ens, member, path, icg) &
+          bind(C, name="sd2d_ens_write_checkpoint")
+       import :: c_int, c_int32_t, c_ptr, c_char
+       type(c_ptr), value :: ens
+       integer(c_int32_t), value :: member
+       character(kind=c_char), intent(in) :: path(*)
+       integer(c_int32_t), intent(in), optional :: icg(*)
+     end function sd2d_ens_write_checkpoint
+
+     integer(c_int) function sd2d_ens_read_checkpoint(ens, member, path) &
+          bind(C, name="sd2d_ens_read_checkpoint")
+       import :: c_int, c_int32_t, c_ptr, c_char
+       type(c_ptr), value :: ens
+       integer(c_int32_t), value :: member
+       character(kind=c_char), intent(in) :: path(*)
+     end function sd2d_ens_read_checkpoint
 
      integer(c_int) function sd2d_ens_step(ens, nsteps) bind(C, name="sd2d_ens_step")
        import :: c_int, c_int32_t, c_ptr

@@ -13,9 +13,20 @@ writes them:
 * resid.out: a header line, then `iter ctime RESIDNORM divB` on every step
   with mod(iter, 10) == 1 (iterations.f90:34-72);
 * restart.%08d.dat on every step with mod(iter, nwrite) == 1 and after the
-  last step (write_data.f90:44-74).
+  last step (write_data.f90:44-74), each with its running-maxima sidecar
+  restart.%08d.dat.max (sd2d_write_checkpoint).  A restart = 1 start from a
+  file that has such a sidecar resumes bit for bit (the maxima restored);
+  from a file without one it resets the maxima to 0 as the reference does
+  (initsetup.f90:154-167).
 
-Not written: the Tecplot output (tecplotter.f90).  The reference's
+* TEC/TEC%07d.%03d.plt (the whole part and the fraction of ctime), the
+  Tecplot binary file of tecplotter.f90, on the same steps as the restart
+  files: the plot mesh refined NRE x NRE per cell, the nodal values formed
+  on the device (sd2d_plot_nodes) and the #!TDV111 writer of
+  sddc_amd/tecplot.py.  Only with ARTIV = 1: the reference allocates muAV,
+  which the file holds, only then (initsetup.f90:36-45).
+
+The reference's
 restart = 0 start (setinitialcond.f90:17-18) perturbs u and v by
 0.02*ran1() from an unseeded generator; here the perturbation is drawn
 from a seeded generator (--seed), so such a start is reproducible but not
@@ -29,6 +40,7 @@ import sys
 
 from . import abi
 from . import mesh as M
+from . import tecplot as TP
 
 
 def _next(it: int, every: int) -> int:
@@ -60,12 +72,17 @@ def run_dir(d: str, precision: int = abi.SD2D_FP64, maxiter: int | None = None,
     g = Sd2d(case, precision=precision)
     try:
         if p.restart == 1:
-            g.read_restart(os.path.join(d, qi.namerestart))
+            rpath = os.path.join(d, qi.namerestart)
+            if os.path.exists(rpath + ".max"):
+                g.read_checkpoint(rpath)
+            else:
+                g.read_restart(rpath)
         else:
             g.set_state(M.freestream_state(case, uv_pert=0.02, seed=seed))
         dg = g.diag()
         it = int(dg["iter"])
-        rows, written = [], []
+        rows, written, tec = [], [], []
+        topo = None
         with open(os.path.join(d, "resid.out"), "w") as fh:
             fh.write(" ITER    ctime     RESIDNORM   divB\n")
         nwrite = int(p.nwrite)
@@ -83,10 +100,21 @@ def run_dir(d: str, precision: int = abi.SD2D_FP64, maxiter: int | None = None,
                     log(f"iter {it} ctime {row[1]:.6e} resid {row[2]:.6e} divB {row[3]:.6e} "
                         f"eigvmax {dg['eigvmax_g']:.6e}")
             if (nwrite > 1 and it % nwrite == 1) or it >= p.maxiter:
+                if p.artiv == 1:
+                    # tecplotter2dsetup, tecplotter2d, tecpost_parallel (iterations.f90:76-81)
+                    if topo is None:
+                        topo = TP.plot_topology(case, int(p.nre))
+                    _, muav, _ = g.get_av()
+                    plotq, plotav = g.plot_nodes(topo, muav)
+                    os.makedirs(os.path.join(d, "TEC"), exist_ok=True)
+                    tn = TP.tec_filename(dg["ctime"])
+                    TP.write_plt(os.path.join(d, "TEC", tn), dg["ctime"], topo, plotq, plotav)
+                    tec.append(tn)
                 fn = f"restart.{it:08d}.dat"
-                g.write_restart(os.path.join(d, fn))
+                g.write_checkpoint(os.path.join(d, fn))
                 written.append(fn)
-        return {"resid_out": rows, "restarts": written, "iter": it, "ctime": g.diag()["ctime"]}
+        return {"resid_out": rows, "restarts": written, "tecplot": tec, "iter": it,
+                "ctime": g.diag()["ctime"]}
     finally:
         g.close()
 

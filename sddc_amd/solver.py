@@ -25,8 +25,9 @@ _lib = None
 
 EXPORTS = ("sd2d_create", "sd2d_destroy", "sd2d_last_error", "sd2d_set_state",
            "sd2d_get_state", "sd2d_set_time", "sd2d_compresid",
-           "sd2d_compresid_q", "sd2d_compresid_noalloc", "sd2d_step", "sd2d_rk_stage", "sd2d_get_diag", "sd2d_get_av",
-           "sd2d_write_restart", "sd2d_read_restart",
+           "sd2d_compresid_q", "sd2d_compresid_noalloc", "sd2d_step", "sd2d_rk_stage", "sd2d_get_diag", "sd2d_get_av", "sd2d_plot_nodes",
+           "sd2d_write_restart", "sd2d_read_restart", "sd2d_write_checkpoint",
+           "sd2d_read_checkpoint", "sd2d_ens_write_checkpoint", "sd2d_ens_read_checkpoint",
            "sd2d_set_profiling", "sd2d_get_profile",
            "sd2d_ens_create", "sd2d_ens_destroy", "sd2d_ens_last_error",
            "sd2d_ens_set_state", "sd2d_ens_get_state", "sd2d_ens_set_time",
@@ -70,7 +71,12 @@ def load_library(path: str = LIBPATH):
     L.sd2d_get_av.argtypes = [vp, vp, vp, C.POINTER(C.c_double)]
     L.sd2d_write_restart.argtypes = [vp, C.c_char_p, vp]
     L.sd2d_read_restart.argtypes = [vp, C.c_char_p]
+    L.sd2d_write_checkpoint.argtypes = [vp, C.c_char_p, vp]
+    L.sd2d_read_checkpoint.argtypes = [vp, C.c_char_p]
+    L.sd2d_ens_write_checkpoint.argtypes = [vp, C.c_int32, C.c_char_p, vp]
+    L.sd2d_ens_read_checkpoint.argtypes = [vp, C.c_int32, C.c_char_p]
     L.sd2d_set_profiling.argtypes = [vp, C.c_int32]
+    L.sd2d_plot_nodes.argtypes = [vp, C.c_int32, C.c_int32, vp, vp, vp, vp, vp]
     L.sd2d_get_profile.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int32)]
     L.sd2d_ens_create.argtypes = [C.c_int32, C.POINTER(abi.Sd2dMesh),
                                   C.POINTER(abi.Sd2dParams), C.POINTER(vp)]
@@ -179,6 +185,20 @@ class Sd2d:
         self._check(self.L.sd2d_get_diag(self.h, C.byref(d)), "sd2d_get_diag")
         return {k: getattr(d, k) for k, _ in abi.Sd2dDiag._fields_}
 
+    def plot_nodes(self, topo, muav=None):
+        """Nodal values of the Tecplot output on the plot mesh `topo`
+        (sddc_amd.tecplot.plot_topology): plotq (9, nnode) and, with the
+        per-cell muAV (2, NCELL) given, plotav (2, nnode)."""
+        nn = topo.nnode
+        xadj = np.ascontiguousarray(topo.xadj, dtype=np.int32)
+        occ = np.ascontiguousarray(topo.contrib, dtype=np.int32)
+        q = np.zeros((9, nn), order="F")
+        av = np.zeros((2, nn), order="F") if muav is not None else None
+        mu = None if muav is None else np.asfortranarray(muav, dtype=np.float64)
+        self._check(self.L.sd2d_plot_nodes(self.h, int(topo.nre), nn, _ptr(xadj), _ptr(occ),
+                                           _ptr(mu), _ptr(q), _ptr(av)), "sd2d_plot_nodes")
+        return q, av
+
     def write_restart(self, path, icg=None):
         """Reference-format restart file of the device state."""
         g = None if icg is None else np.ascontiguousarray(icg, dtype=np.int32)
@@ -188,6 +208,18 @@ class Sd2d:
     def read_restart(self, path):
         """Load a reference-format restart file (maxima reset to 0)."""
         self._check(self.L.sd2d_read_restart(self.h, str(path).encode()), "sd2d_read_restart")
+
+    def write_checkpoint(self, path, icg=None):
+        """Reference-format restart file plus the running-maxima sidecar
+        <path>.max, from which a run resumes bit for bit."""
+        g = None if icg is None else np.ascontiguousarray(icg, dtype=np.int32)
+        self._check(self.L.sd2d_write_checkpoint(self.h, str(path).encode(), _ptr(g)),
+                    "sd2d_write_checkpoint")
+
+    def read_checkpoint(self, path):
+        """Load a checkpoint: state, iter, ctime and the running maxima."""
+        self._check(self.L.sd2d_read_checkpoint(self.h, str(path).encode()),
+                    "sd2d_read_checkpoint")
 
     def set_check_finite(self, every: int):
         """Opt-in NaN/Inf check of the state every `every` steps (0: off);
@@ -300,6 +332,17 @@ class Sd2dEnsemble:
         self._check(self.L.sd2d_ens_get_time(self.h, i, C.byref(t), C.byref(it)),
                     "sd2d_ens_get_time")
         return t.value, it.value
+
+    def write_checkpoint(self, i, path, icg=None):
+        """Member i's checkpoint (restart file plus maxima sidecar)."""
+        g = None if icg is None else np.ascontiguousarray(icg, dtype=np.int32)
+        self._check(self.L.sd2d_ens_write_checkpoint(self.h, int(i), str(path).encode(), _ptr(g)),
+                    "sd2d_ens_write_checkpoint")
+
+    def read_checkpoint(self, i, path):
+        """Load a checkpoint into member i (state, maxima, iter, ctime)."""
+        self._check(self.L.sd2d_ens_read_checkpoint(self.h, int(i), str(path).encode()),
+                    "sd2d_ens_read_checkpoint")
 
     def step(self, nsteps):
         self._check(self.L.sd2d_ens_step(self.h, int(nsteps)), "sd2d_ens_step")

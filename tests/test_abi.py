@@ -274,6 +274,11 @@ def test_fortran_module_declares_every_entry_point():
     src = open(os.path.join(os.path.dirname(HERE), "sddc_amd", "fortran", "sd2d_c.f90")).read()
     bound = sorted(set(re.findall(r'bind\(C,\s*name="(sd2d_\w+)"\)', src)))
     assert bound == declared_functions()
+    # and the module (private by default) makes every one of them public
+    m = re.search(r"^\s*public ::\s*(sd2d_.*?)\n\s*\n", src, re.S | re.M)
+    assert m, "no public statement of the entry points"
+    public = set(re.findall(r"\b(sd2d_\w+)\b", m.group(1)))
+    assert set(bound) <= public, sorted(set(bound) - public)
 
 
 def test_noalloc_limit_matches_header():

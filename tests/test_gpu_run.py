@@ -36,3 +36,35 @@ def test_run_driver_reference_deck_c1(tmp_path):
     lines = (tmp_path / "resid.out").read_text().splitlines()
     assert len(lines) == 3
     np.testing.assert_allclose([float(x) for x in lines[2].split()[1:]], rows[1, 1:], rtol=1e-15)
+
+
+def test_run_driver_resumes_bitwise(tmp_path):
+    """The deck run for 11 steps, against the same deck run for 6 steps and
+    then restarted from the restart file it wrote at step 6 (with its
+    running-maxima sidecar) up to 11: the final restart files are byte for
+    byte the same, and so are the resid.out rows after the restart."""
+    import os
+    import shutil
+    print("[gpu-run] run driver resume from a checkpoint", flush=True)
+    case, Q0, fx = specs.load("c1_cylinder")
+    full, part = tmp_path / "full", tmp_path / "part"
+    for d, mi in ((full, 11), (part, 6)):
+        d.mkdir()
+        refrun.write_case(str(d), case, Q0, 0.0, maxiter=mi, nwrite=100)
+    R.run_dir(str(full))
+    R.run_dir(str(part))
+    assert os.path.exists(part / "restart.00000006.dat.max")
+    # restart = 1 from restart.00000006.dat, MAXITER 11
+    shutil.copy(part / "restart.00000006.dat", part / "ic.dat")
+    shutil.copy(part / "restart.00000006.dat.max", part / "ic.dat.max")
+    lines = (part / "QUAD.INP").read_text().split("\n")
+    lines[17] = "11"                                  # MAXITER (readinput.f90:25)
+    (part / "QUAD.INP").write_text("\n".join(lines))
+    out = R.run_dir(str(part))
+    assert out["iter"] == 11
+    assert (part / "restart.00000011.dat").read_bytes() == (full / "restart.00000011.dat").read_bytes()
+    assert (part / "restart.00000011.dat.max").read_bytes() == \
+        (full / "restart.00000011.dat.max").read_bytes()
+    r_full = (full / "resid.out").read_text().splitlines()
+    r_part = (part / "resid.out").read_text().splitlines()
+    assert r_part[-1] == r_full[-1]                   # the step-11 row
