@@ -12,8 +12,11 @@ Prints one JSON line:
   roofline    the fused stage kernel against HBM bandwidth: algorithmic
               bytes per launch (NDOF * B_alg) / average launch time from
               HIP events on the library's stream
-  cpu_baseline  the CPU restatement of the reference (OpenMP) on one step
-              of the same workload on this host's cores
+  cpu_baseline  the reference Fortran program itself (oracle/_ref/sd2d_MHD_O3,
+              flang -O3, MPI ranks on this host's cores) on a sample of
+              the same workload: (T(2K) - T(K)) / K per step from two runs
+              (kind "reference"); cpu_baseline_port: the C restatement of
+              the reference (OpenMP) on a few steps of the same workload
 """
 from __future__ import annotations
 
@@ -107,7 +110,11 @@ def run_c4(steps: int, warmup: int, tune_steps: int = 2) -> dict:
         wall = time.perf_counter() - t0
         states[prec] = [ens.get_state(i)[0] for i in range(len(mem))]
         ens.close()
+        fin = [bool(np.all(np.isfinite(q))) for q in states[prec]]
+        dof_fin = sum(cases.dof(c) for (c, _, _), f in zip(mem, fin) if f)
         res[prec] = {"value": ndof * 5 * steps / wall, "wall_s": wall,
+                     "finite_members": int(sum(fin)),
+                     "value_finite_members": dof_fin * 5 * steps / wall,
                      "tile_shapes": {str(n): ("strips" if v == abi.SD2D_TILE_STRIPS else "blocks")
                                      for n, v in shapes.items()},
                      "tune_s": tt}
@@ -149,7 +156,11 @@ def run_c4(steps: int, warmup: int, tune_steps: int = 2) -> dict:
             / (ndof * 5 * steps / seq),
             "tile_shapes_mixed": res[abi.SD2D_MIXED]["tile_shapes"],
             "tile_shapes_fp64": res[abi.SD2D_FP64]["tile_shapes"],
-            "finite_members": nfin,
+            "finite_members_fp64": res[abi.SD2D_FP64]["finite_members"],
+            "finite_members_mixed": res[abi.SD2D_MIXED]["finite_members"],
+            "mixed_value_finite_members": res[abi.SD2D_MIXED]["value_finite_members"],
+            "fp64_value_finite_members": res[abi.SD2D_FP64]["value_finite_members"],
+            "finite_in_both": nfin,
             "l2_rel_mixed_vs_fp64_worst_member": [float(x) for x in l2]}
 
 
@@ -260,6 +271,11 @@ def main() -> int:
     ap.add_argument("--cpu-steps", type=int, default=3,
                     help="steps of the CPU baseline sample (0 = skip); 3 C2 steps are "
                          "about 10 s of the oracle on 16 host threads")
+    ap.add_argument("--cpu-ref-steps", type=int, default=2,
+                    help="K of the reference program's (T(2K) - T(K)) / K timing (0 = skip)")
+    ap.add_argument("--cpu-ref-ranks", type=int, default=0,
+                    help="MPI ranks of the reference program (0: this host's CPU share, "
+                         "OMP_NUM_THREADS or the core count, at most 16)")
     ap.add_argument("--tile-shape", default="strips", choices=["strips", "blocks"],
                     help="cells of one workgroup: mesh-order strips (default) or compact blocks")
     ap.add_argument("--no-mixed", action="store_true",
@@ -326,7 +342,10 @@ def main() -> int:
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (seeded free stream with 1e-3 density perturbation)",
+        "data": "synthetic: free stream (Mach 1.5, Alfven Mach 4 at 30 degrees) with a seeded "
+                "1e-3 uniform density perturbation, started at rest on the wall with the "
+                "velocity rising to the free stream over one radius (an impulsive start turns "
+                "NaN within three steps, in the reference too)",
         "config": {"workload": desc, "ncell": case.ncell, "N": N, "dof": ndof,
                    "dt": case.params.dt, "rk": "RK_S5P3", "av": True,
                    "norm_every": 10, "tile_shape": args.tile_shape},
@@ -391,6 +410,26 @@ def main() -> int:
                                          f"workload ({5 * args.cpu_steps} residual "
                                          f"evaluations), {tcpu:.1f}s"}
         o.close()
+    if args.cpu_ref_steps > 0:
+        from oracle import time_reference as TR
+        if TR.available():
+            ranks = args.cpu_ref_ranks or min(16, int(os.environ.get("OMP_NUM_THREADS", "0"))
+                                              or os.cpu_count() or 1)
+            print(f"[bench] reference program: {ranks} MPI ranks, {args.cpu_ref_steps} and "
+                  f"{2 * args.cpu_ref_steps} steps", file=sys.stderr, flush=True)
+            r = TR.time_reference(case, Q0, ranks, args.cpu_ref_steps)
+            if "cpu_baseline" in res:
+                res["cpu_baseline_port"] = res["cpu_baseline"]
+            res["cpu_baseline"] = {
+                "value": r["dof_stage_per_s"], "unit": "DOF-stage/s", "cores": ranks,
+                "kind": "reference",
+                "sample": f"the reference program (flang -O3, {ranks} MPI ranks, cells split "
+                          f"into contiguous ring ranges) on the same workload for "
+                          f"{r['steps'][0]} and {r['steps'][1]} RK_S5P3 steps: "
+                          f"{r['elapsed_s'][0]:.1f}s and {r['elapsed_s'][1]:.1f}s, "
+                          f"{r['s_per_step']:.2f}s per step"}
+            print(f"[bench] reference program: {r['s_per_step']:.2f} s per step", file=sys.stderr,
+                  flush=True)
     print(json.dumps(res), flush=True)
     return 0
 
