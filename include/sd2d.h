@@ -73,8 +73,17 @@ enum { SD2D_TILE_DEFAULT = 0, SD2D_TILE_STRIPS = 1, SD2D_TILE_BLOCKS = 2 };
  *   lmat,mmat Lmat(N+1,N), Mmat(N+1,N) (initsetup.f90:262-319)
  *   xs,xs_low Xs(N), Xs_low(N-1)  (initsetup.f90:171-259, :43-44)
  *   jac,s1,s2 Jac(N,N,NCELL), S1(2,2,N+1,N,NCELL), S2(2,2,N,N+1,NCELL)
- *                                 (calcjacob.f90:55,85-89,121-125)
+ *                                 (calcjacob.f90:55,85-89,121-125), from
+ *                                 4-node cells or, CURVE_WALL = 1, from the
+ *                                 8 nodes of every cell (calcjacob.f90:26-53)
  * The arrays are copied; the caller may free them after sd2d_create.
+ * The two cells of an interior or periodic face must give its face points
+ * one normal up to sign, to 1e-9 of its length: the reference evaluates
+ * the face with the left cell's normal (interfaceflux_all.f90:47,54,
+ * cyclocflux_all.f90:47,54).  Where the right cell's own S1/S2 vector
+ * differs from it by rounding (8-node cells), the device uses the left's
+ * at that face point; a larger difference (a non-conforming face) is
+ * refused with SD2D_EUNSUPPORTED.
  */
 typedef struct sd2d_mesh {
   int32_t N;

@@ -3,7 +3,8 @@
 ! oracle/_ref/.  Test infrastructure only: it runs in the build container,
 ! never on the GPU machine, and nothing in the product links it.
 !
-! It performs the reference's set-up chain (main.f90:16-46), dumps the
+! It performs the reference's set-up chain (main.f90:16-46, the .vrt8
+! nodes read when CURVE_WALL = 1 as main.f90:28 does), dumps the
 ! set-up arrays, then advances the state:
 !   - the first NSTAGED steps stage by stage, calling the reference's
 !     compresid and applying the RK_S5P3 stage updates
@@ -51,6 +52,7 @@ program ref_driver
     call CHECK_MESH
     call READ_METIS
     call GLOB2LOCAL
+    if (CURVE_WALL==1) call READ_VRT8_DATA
     call READ_BND_DATA
     call CONNECTIVITY
     call CONNECT_BDRY

@@ -3,6 +3,7 @@
 // built once from the arrays module setup2d holds after INIT_SETUP.
 #pragma once
 #include <algorithm>
+#include <cmath>
 #include <cstdlib>
 #include <map>
 #include <cstdint>
@@ -69,15 +70,38 @@ inline double lagrange(const double *x, int n, int i, double t) {
 
 }  // namespace hostdetail
 
-// Per-cell face table and the check that the left cell's normal equals +-
-// the right cell's own metric vector bit for bit at every face point.
+// A face point whose right cell takes the left cell's normal (times the
+// face's sign) in the device metric table.
+struct NormalAdopt {
+  int cell, k, nfp;   // right cell (0-based), its local face (0-based), own point (1-based)
+  double nx, ny;
+};
+
+// Two normals of one face point that agree to this fraction of their
+// length differ only by rounding: both cells map the shared edge through
+// the same nodes (an 8-node edge through its three nodes, whose
+// derivatives the two cells form in different orders).  A non-conforming
+// pairing differs at O(1).
+constexpr double NORMAL_RTOL = 1e-9;
+
+// Per-cell face table.  Each cell evaluates its faces itself, so the two
+// cells of an interior or periodic face must use one normal up to sign:
+// the reference forms the face flux with the left cell's normal only
+// (interfaceflux_all.f90:47,54, cyclocflux_all.f90:47,54).  Where the
+// right cell's own vector equals -+ the left's bit for bit (bilinear
+// cells) nothing changes; where they differ by rounding (curved 8-node
+// cells) the face point is listed in `adopt`, and build_host_tables puts
+// the left's normal into the right cell's metric slot.  A pair that
+// differs beyond NORMAL_RTOL is refused.
 inline int build_face_table(const sd2d_mesh *m, int CPW, const std::vector<int> &c2s,
-                            std::vector<int> &nb, std::vector<int> &info, std::string &err) {
+                            std::vector<int> &nb, std::vector<int> &info,
+                            std::vector<NormalAdopt> &adopt, std::string &err) {
   using hostdetail::HostMesh;
   const int N = m->N, nc = m->ncell, N2 = N * N;
   HostMesh hm{N, nc, m->nface, m};
   nb.assign((size_t)nc * 4, 0);
   info.assign((size_t)nc * 4, -1);
+  adopt.clear();
   auto chk_face = [&](int f) { return f >= 1 && f <= m->nface; };
   auto base_of = [&](int cell) {
     const size_t slot = c2s[cell], tile = slot / CPW, ci = slot % CPW;
@@ -96,10 +120,16 @@ inline int build_face_table(const sd2d_mesh *m, int CPW, const std::vector<int> 
       int s;
       if (lx == rx && ly == ry) s = 0;
       else if (lx == -rx && ly == -ry) s = 1;
-      else
-        return bad(SD2D_EUNSUPPORTED,
-                   "face normals of neighbouring cells differ beyond sign "
-                   "(non-conforming or curved faces are not supported)");
+      else {
+        const double tol = NORMAL_RTOL * std::max(std::hypot(lx, ly), std::hypot(rx, ry));
+        if (std::fabs(lx - rx) <= tol && std::fabs(ly - ry) <= tol) s = 0;
+        else if (std::fabs(lx + rx) <= tol && std::fabs(ly + ry) <= tol) s = 1;
+        else
+          return bad(SD2D_EUNSUPPORTED,
+                     "face normals of neighbouring cells differ beyond sign and rounding "
+                     "(non-conforming faces are not supported)");
+        adopt.push_back({cr, kr - 1, N - nfp + 1, s ? -lx : lx, s ? -ly : ly});
+      }
       if (sneg >= 0 && s != sneg)
         return bad(SD2D_EUNSUPPORTED, "face normal sign varies along a face");
       sneg = s;
@@ -339,7 +369,8 @@ inline int build_host_tables(const sd2d_mesh *m, const sd2d_params *p, HostTable
              t.c2s, t.s2c);
   t.ntile = (int)(t.s2c.size() / CPW);
   std::vector<int> nb, info;
-  int r = build_face_table(m, CPW, t.c2s, nb, info, err);
+  std::vector<NormalAdopt> adopt;
+  int r = build_face_table(m, CPW, t.c2s, nb, info, adopt, err);
   if (r) return r;
   const int nt = t.ntile;
   // the tiles are renumbered in launch order, so that a workgroup finds its
@@ -358,7 +389,7 @@ inline int build_host_tables(const sd2d_mesh *m, const sd2d_params *p, HostTable
     }
     nb.clear();
     info.clear();
-    r = build_face_table(m, CPW, t.c2s, nb, info, err);
+    r = build_face_table(m, CPW, t.c2s, nb, info, adopt, err);
     if (r) return r;
   }
   t.ft.assign((size_t)nt * 8 * CPW, 0);
@@ -406,6 +437,30 @@ inline int build_host_tables(const sd2d_mesh *m, const sd2d_params *p, HostTable
       for (int is = 0; is < N; ++is)
         put_tile<double>(t.met, NM, CPW, t.c2s[cell], 4 * N * (N + 1) + js + N * is,
                          1.0 / m->jac[is + N * (js + (size_t)N * cell)]);
+  }
+  // the right cell of a face whose normals differ by rounding takes the
+  // left cell's normal at that face point (face2fpconnec, as in
+  // HostMesh::normal); everything else of the right cell keeps its own
+  // metrics, so its divB/Jz weights at that point and its viscous terms use
+  // a vector within rounding of the reference's own
+  for (const NormalAdopt &a : adopt) {
+    int ifp, jfp;
+    switch (a.k) {
+      case 0: ifp = a.nfp; jfp = 1; break;
+      case 1: ifp = N + 1; jfp = a.nfp; break;
+      case 2: ifp = N - a.nfp + 1; jfp = N + 1; break;
+      default: ifp = 1; jfp = N - a.nfp + 1; break;
+    }
+    const int sl = t.c2s[a.cell];
+    if (a.k == 1 || a.k == 3) {
+      const int rfp = ifp - 1, js = jfp - 1;
+      put_tile<double>(t.met, NM, CPW, sl, 0 + rfp + (N + 1) * js, a.nx);
+      put_tile<double>(t.met, NM, CPW, sl, N * (N + 1) + rfp + (N + 1) * js, a.ny);
+    } else {
+      const int rfp = jfp - 1, is = ifp - 1;
+      put_tile<double>(t.met, NM, CPW, sl, 2 * N * (N + 1) + rfp + (N + 1) * is, a.nx);
+      put_tile<double>(t.met, NM, CPW, sl, 3 * N * (N + 1) + rfp + (N + 1) * is, a.ny);
+    }
   }
   // viscous path: S1(2,1:2,rfp,js) and S2(1,1:2,is,rfp) at the two ends of
   // every line, used by the adiabatic-wall gradient ghost (bcflux.f90:547-571)

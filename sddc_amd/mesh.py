@@ -42,6 +42,10 @@ class Mesh:
     cells: (NC,4) 1-based vertex ids (the .cel columns IV1..IV4)
     xv, yv: (NV,) vertex coordinates (.vrt)
     bnd: list of (iv1, iv2, type) boundary edges, 1-based (.bnd)
+    x8: (NC,8,2) or None: the eight nodes of every cell (.vrt8, read when
+        CURVE_WALL = 1, readGRIDinfo.f90:86-117): the corners in the
+        cell's vertex order, then the midside nodes of its edges 1-2, 2-3,
+        3-4 and 4-1 (the node numbering drawn at mapder.f90:107-116)
     """
     cells: np.ndarray
     xv: np.ndarray
@@ -50,6 +54,7 @@ class Mesh:
     dxcycl: float = 0.0
     dycycl: float = 0.0
     name: str = "mesh"
+    x8: np.ndarray | None = None
 
     @property
     def ncell(self) -> int:
@@ -57,7 +62,7 @@ class Mesh:
 
 
 def ogrid(nr: int, ntheta: int, r0: float = 0.5, r1: float = 15.0,
-          farfield: str = "split") -> Mesh:
+          farfield: str = "split", curved: bool = False) -> Mesh:
     """O-grid around a cylinder of radius r0 out to r1.
 
     Radial spacing is geometric, r_i = r0*q**i with q = (r1/r0)**(1/nr).
@@ -67,6 +72,12 @@ def ogrid(nr: int, ntheta: int, r0: float = 0.5, r1: float = 15.0,
     The inner circle is WALL.  The outer circle is INLE where the face
     midpoint has x < 0 and OUTL elsewhere (farfield="split"), or INLE
     everywhere (farfield="inle").
+
+    curved=True adds the 8-node description of every cell (Mesh.x8, for
+    CURVE_WALL = 1): the midside node of an arc edge lies on its circle at
+    the middle angle, that of a radial edge at the mean radius of its ends,
+    so every cell follows the circles exactly to second order.  Nodes
+    shared by two cells are computed once, so both cells see the same bits.
     """
     q = (r1 / r0) ** (1.0 / nr)
     r = np.array([r0 * q ** i for i in range(nr + 1)])
@@ -93,7 +104,20 @@ def ogrid(nr: int, ntheta: int, r0: float = 0.5, r1: float = 15.0,
         xm = 0.5 * (X[a - 1] + X[b - 1])
         t = "INLE" if (farfield == "inle" or xm < 0.0) else "OUTL"
         bnd.append((a, b, t))
-    return Mesh(cells, X, Y, bnd, name=f"ogrid{nr}x{ntheta}")
+    if not curved:
+        return Mesh(cells, X, Y, bnd, name=f"ogrid{nr}x{ntheta}")
+    thm = 2.0 * np.pi * (np.arange(ntheta) + 0.5) / ntheta      # middle angle of column j
+    rm = 0.5 * (r[:-1] + r[1:])                                  # mean radius of ring i
+    x8 = np.empty((nr * ntheta, 8, 2))
+    for k in range(4):
+        x8[:, k, 0] = X[cells[:, k] - 1]
+        x8[:, k, 1] = Y[cells[:, k] - 1]
+    j1 = (jj + 1) % ntheta
+    x8[:, 4, 0] = rm[ii] * np.cos(th[jj]); x8[:, 4, 1] = rm[ii] * np.sin(th[jj])          # 1-2
+    x8[:, 5, 0] = r[ii + 1] * np.cos(thm[jj]); x8[:, 5, 1] = r[ii + 1] * np.sin(thm[jj])  # 2-3
+    x8[:, 6, 0] = rm[ii] * np.cos(th[j1]); x8[:, 6, 1] = rm[ii] * np.sin(th[j1])          # 3-4
+    x8[:, 7, 0] = r[ii] * np.cos(thm[jj]); x8[:, 7, 1] = r[ii] * np.sin(thm[jj])          # 4-1
+    return Mesh(cells, X, Y, bnd, name=f"ogrid_curved{nr}x{ntheta}", x8=x8)
 
 
 def periodic_square(nx: int, ny: int, lx: float = 2 * math.pi,
@@ -199,6 +223,7 @@ class RunParams:
     bxinf: float = 0.3
     byinf: float = 0.0
     alpha: float = 2.0  # initsetup.f90:165
+    curve_wall: int = 0  # CURVE_WALL: 1 = 8-node cells from the .vrt8 file
 
 
 def _f(x: float) -> str:
@@ -209,7 +234,7 @@ def write_quad_inp(path: str, p: RunParams, names=("mesh.cel", "mesh.vrt",
                    "mesh.vrt8", "mesh.bnd", "mesh.mts", "ic.dat")) -> None:
     """QUAD.INP in the positional order of readinput.f90:8-43."""
     L = ["SD2D-MHD input", "files", *names, "N CURVE vismode mu eta0",
-         str(p.N), "0", str(p.vismode), _f(p.mu), _f(p.eta0),
+         str(p.N), str(p.curve_wall), str(p.vismode), _f(p.mu), _f(p.eta0),
          "k_stage dt MAXITER", str(p.k_stage), _f(p.dt), str(p.maxiter),
          "DXCYCL DYCYCL", _f(p.dxcycl), _f(p.dycycl), "ARTIV", str(p.artiv),
          "NRE nwrite", str(p.nre), str(p.nwrite), "restart",
@@ -231,8 +256,23 @@ def write_mesh_files(dirpath: str, m: Mesh, stem: str = "mesh") -> None:
     with open(os.path.join(dirpath, stem + ".bnd"), "w") as fh:
         for i, (a, b, t) in enumerate(m.bnd):
             fh.write(f"{i+1} {a} {b} 1 1 {t}\n")
-    open(os.path.join(dirpath, stem + ".vrt8"), "w").close()
+    with open(os.path.join(dirpath, stem + ".vrt8"), "w") as fh:
+        if m.x8 is not None:     # eight "x y" records per cell (readGRIDinfo.f90:97-101)
+            for c in range(m.ncell):
+                for k in range(8):
+                    fh.write(f"{m.x8[c, k, 0]:.17E} {m.x8[c, k, 1]:.17E}\n")
     open(os.path.join(dirpath, stem + ".mts"), "w").close()
+
+
+def read_vrt8(path: str, ncell: int) -> np.ndarray:
+    """The .vrt8 file as READ_VRT8_DATA reads it (readGRIDinfo.f90:94-113):
+    eight list-directed (x, y) records per cell, cells in global order.
+    Returns x8 (ncell, 8, 2)."""
+    rec = _records(path)
+    if len(rec) < 8 * ncell:
+        raise ValueError(f"{path}: {len(rec)} records, {8 * ncell} needed for {ncell} cells")
+    v = np.array([[_ld_float(r[0]), _ld_float(r[1])] for r in rec[:8 * ncell]])
+    return v.reshape(ncell, 8, 2)
 
 
 def write_restart(path: str, Q: np.ndarray, ctime: float = 0.0,
@@ -337,7 +377,7 @@ def read_quad_inp(path: str) -> QuadInput:
     p = RunParams(N=N, vismode=vismode, mu=mu, eta0=eta0, k_stage=k_stage, dt=dt,
                   maxiter=maxiter, dxcycl=dxcycl, dycycl=dycycl, artiv=artiv, nre=nre,
                   nwrite=nwrite, restart=restart, rinf=rinf, pinf=pinf, uinf=uinf,
-                  vinf=vinf, bxinf=bxinf, byinf=byinf)
+                  vinf=vinf, bxinf=bxinf, byinf=byinf, curve_wall=curve)
     return QuadInput(p, *names, curve_wall=curve, restart_ord=restart_ord)
 
 
@@ -607,20 +647,43 @@ def _dmdx(psi, eta):
             (-eta, 1.0 - psi))
 
 
-def metrics(cells: np.ndarray, xv: np.ndarray, yv: np.ndarray, N: int):
+def _dmdx8(psi, eta):
+    """Serendipity 8-node shape-function derivatives (MAPDER_8_NODE,
+    mapder.f90:117-253, the same expressions at solution and flux points):
+    rows k=1..8, columns d/dpsi, d/deta."""
+    return (((1.0 - eta) * (4.0 * psi + 2.0 * eta - 3.0), (1.0 - psi) * (4.0 * eta + 2.0 * psi - 3.0)),
+            ((1.0 - eta) * (4.0 * psi - 2.0 * eta - 1.0), psi * (4.0 * eta - 2.0 * psi - 1.0)),
+            (eta * (4.0 * psi + 2.0 * eta - 3.0), psi * (4.0 * eta + 2.0 * psi - 3.0)),
+            (eta * (4.0 * psi - 2.0 * eta - 1.0), (1.0 - psi) * (4.0 * eta - 2.0 * psi - 1.0)),
+            (4.0 * (1.0 - eta) * (1.0 - 2.0 * psi), 4.0 * psi * (psi - 1.0)),
+            (4.0 * eta * (1.0 - eta), 4.0 * psi * (1.0 - 2.0 * eta)),
+            (4.0 * eta * (1.0 - 2.0 * psi), 4.0 * psi * (1.0 - psi)),
+            (4.0 * eta * (eta - 1.0), 4.0 * (1.0 - psi) * (1.0 - 2.0 * eta)))
+
+
+def metrics(cells: np.ndarray, xv: np.ndarray, yv: np.ndarray, N: int,
+            x8: np.ndarray | None = None):
     """Jac(N,N,NC), S1(2,2,N+1,N,NC), S2(2,2,N,N+1,NC) in Fortran order.
 
-    cells must already be CCW (orient_ccw).  Sums run over k=1..4 from 0.0
-    as in calcjacob.f90:63-69, so the values are bitwise those of the
-    reference (compiled without FMA contraction)."""
+    Bilinear cells (x8 None): cells must already be CCW (orient_ccw) and the
+    sums run over the four vertices k=1..4 from 0.0 as in
+    calcjacob.f90:39-46.  8-node cells (CURVE_WALL = 1): the sums run over
+    the eight nodes x8 (Mesh.x8) with the derivatives of MAPDER_8_NODE
+    (calcjacob.f90:26-29,47-53).  Either way the values are bitwise those of
+    the reference (compiled without FMA contraction)."""
     xs, xf = setsandpoints(N)
     nc = len(cells)
-    XV = xv[cells - 1]; YV = yv[cells - 1]   # (nc,4)
+    if x8 is None:
+        XV = xv[cells - 1]; YV = yv[cells - 1]   # (nc,4)
+        dmdx, K = _dmdx, 4
+    else:
+        XV = np.ascontiguousarray(x8[:, :, 0]); YV = np.ascontiguousarray(x8[:, :, 1])
+        dmdx, K = _dmdx8, 8
 
     def derivs(psi, eta):
-        d = _dmdx(psi, eta)
+        d = dmdx(psi, eta)
         xp = np.zeros(nc); yp = np.zeros(nc); xe = np.zeros(nc); ye = np.zeros(nc)
-        for k in range(4):
+        for k in range(K):
             xp = xp + d[k][0] * XV[:, k]
             yp = yp + d[k][0] * YV[:, k]
             xe = xe + d[k][1] * XV[:, k]
@@ -647,23 +710,50 @@ def metrics(cells: np.ndarray, xv: np.ndarray, yv: np.ndarray, N: int):
     return jac, s1, s2
 
 
+def base_functions(xi, eta, nodes: int = 4):
+    """MapBaseFunc (mapder.f90:79-87) or MapBaseFunc8 (mapder.f90:91-102)
+    at (xi, eta): the 4 or 8 node weights of xyCoor_atGps."""
+    if nodes == 4:
+        return ((1.0 - xi) * (1.0 - eta), (1.0 - eta) * xi, xi * eta, (1.0 - xi) * eta)
+    return ((1.0 - xi) * (1.0 - eta) * (-2.0 * xi - 2.0 * eta + 1.0),
+            xi * (1.0 - eta) * (2.0 * xi - 2.0 * eta - 1.0),
+            xi * eta * (2.0 * xi + 2.0 * eta - 3.0),
+            (1.0 - xi) * eta * (-2.0 * xi + 2.0 * eta - 1.0),
+            4.0 * (1.0 - eta) * (1.0 - xi) * xi,
+            4.0 * eta * (1.0 - eta) * xi,
+            4.0 * (1.0 - xi) * xi * eta,
+            4.0 * (1.0 - eta) * (1.0 - xi) * eta)
+
+
 def solution_coords(cells: np.ndarray, xv: np.ndarray, yv: np.ndarray,
-                    N: int):
+                    N: int, x8: np.ndarray | None = None):
     """Physical coordinates of the solution points, (2,N,N,NC) as XXsolu
-    (initsetup.f90:91-106, xyCoor_atGps mapder.f90:255-273)."""
+    (initsetup.f90:91-106, xyCoor_atGps mapder.f90:255-273), from the four
+    vertices or, for 8-node cells, from x8 (Mesh.x8).  (The reference
+    leaves the node array of this loop unset for 8-node cells,
+    initsetup.f90:88-89, so its restart = 0 start on such a mesh reads
+    undefined coordinates; only its restart = 1 start is defined there.)"""
     xs, _ = setsandpoints(N)
-    XV = xv[cells - 1]; YV = yv[cells - 1]
+    if x8 is None:
+        XV = xv[cells - 1]; YV = yv[cells - 1]; K = 4
+    else:
+        XV = x8[:, :, 0]; YV = x8[:, :, 1]; K = 8
     out = np.zeros((2, N, N, len(cells)), order="F")
     for js in range(N):
         for is_ in range(N):
-            x, e = xs[is_], xs[js]
-            f = ((1.0 - x) * (1.0 - e), (1.0 - e) * x, x * e, (1.0 - x) * e)
+            f = base_functions(xs[is_], xs[js], K)
             px = np.zeros(len(cells)); py = np.zeros(len(cells))
-            for k in range(4):
+            for k in range(K):
                 px = px + XV[:, k] * f[k]
                 py = py + YV[:, k] * f[k]
             out[0, is_, js] = px; out[1, is_, js] = py
     return out
+
+
+def case_coords(case: "Case") -> np.ndarray:
+    """solution_coords of a case, with its 8-node cells when it has them."""
+    return solution_coords(case.cells, case.mesh.xv, case.mesh.yv, case.N,
+                           case.mesh.x8 if case.params.curve_wall == 1 else None)
 
 
 # --------------------------------------------------------------------------
@@ -698,9 +788,21 @@ def build_case(m: Mesh, p: RunParams) -> Case:
     N = p.N
     topo = build_topology(m, p.dxcycl, p.dycycl)
     cells = orient_ccw(m)
+    x8 = None
+    if p.curve_wall == 1:
+        if m.x8 is None or m.x8.shape != (m.ncell, 8, 2):
+            raise ValueError("CURVE_WALL = 1 needs the 8 nodes of every cell (Mesh.x8)")
+        if not np.array_equal(cells, np.asarray(m.cells)):
+            # CHECK_MESH turns a clockwise cell's vertices round but leaves its
+            # .vrt8 nodes as read, so the 8-node map of that cell would be
+            # mirrored (negative Jacobian)
+            raise ValueError("CURVE_WALL = 1 needs every cell counter-clockwise in the .cel file")
+        x8 = m.x8
+    elif p.curve_wall != 0:
+        raise ValueError(f"CURVE_WALL must be 0 or 1, not {p.curve_wall}")
     xs, xf, L, M = lmat_mmat(N)
     xl, _ = setsandpoints(N - 1) if N > 1 else (np.zeros(0), None)
-    jac, s1, s2 = metrics(cells, m.xv, m.yv, N)
+    jac, s1, s2 = metrics(cells, m.xv, m.yv, N, x8)
     return Case(m, p, cells, topo, xs, xf, L, M, np.asarray(xl), jac, s1, s2)
 
 
@@ -735,7 +837,7 @@ def freestream_state(case: Case, rho_pert: float = 0.0, uv_pert: float = 0.0,
         rho = rho * (1.0 + rho_pert * rng.uniform(-1.0, 1.0, size=shp))
     u = np.full(shp, p.uinf); v = np.full(shp, p.vinf)
     if wall_ramp is not None:
-        xy = solution_coords(case.cells, case.mesh.xv, case.mesh.yv, N)
+        xy = case_coords(case)
         r = np.hypot(xy[0], xy[1])
         s = 1.0 - np.exp(-(np.maximum(r - wall_ramp[0], 0.0) / wall_ramp[1]) ** 2)
         u = u * s; v = v * s
@@ -752,7 +854,7 @@ def orszag_tang_state(case: Case, rho_pert: float = 1e-4, seed: int = 0
                       ) -> np.ndarray:
     """Orszag-Tang vortex on [0,2pi]^2: rho=25/9, p=5/3, u=(-sin y, sin x),
     B=(-sin y, sin 2x), plus a seeded normal perturbation of rho."""
-    xy = solution_coords(case.cells, case.mesh.xv, case.mesh.yv, case.N)
+    xy = case_coords(case)
     x, y = xy[0], xy[1]
     rng = np.random.default_rng(seed)
     rho = np.full(x.shape, 25.0 / 9.0)
@@ -770,7 +872,7 @@ def smooth_perturbed_state(case: Case, amp: float = 0.05, seed: int = 0
     (including w, Bz and psi), so that every term of the residual is
     exercised and no residual component cancels to round-off."""
     p = case.params
-    xy = solution_coords(case.cells, case.mesh.xv, case.mesh.yv, case.N)
+    xy = case_coords(case)
     x, y = xy[0], xy[1]
     rng = np.random.default_rng(seed)
     k = rng.uniform(0.5, 2.0, size=(9, 2)); ph = rng.uniform(0, 2 * np.pi, 9)

@@ -30,7 +30,11 @@ The reference's
 restart = 0 start (setinitialcond.f90:17-18) perturbs u and v by
 0.02*ran1() from an unseeded generator; here the perturbation is drawn
 from a seeded generator (--seed), so such a start is reproducible but not
-the reference's own draw.  CURVE_WALL = 1 (8-node cells) is refused.
+the reference's own draw.  CURVE_WALL = 1 reads the 8 nodes of every cell
+from the .vrt8 file named in QUAD.INP (readGRIDinfo.f90:86-117) and builds
+the curved metrics from them; such a deck must start from a restart file,
+as the reference's restart = 0 start reads coordinates it never sets for
+8-node cells (initsetup.f90:82-89).
 """
 from __future__ import annotations
 
@@ -61,13 +65,18 @@ def run_dir(d: str, precision: int = abi.SD2D_FP64, maxiter: int | None = None,
     p = qi.params
     if maxiter is not None:
         p.maxiter = int(maxiter)
-    if qi.curve_wall != 0:
-        raise NotImplementedError("CURVE_WALL = 1 (8-node cells) is not supported")
+    if qi.curve_wall not in (0, 1):
+        raise ValueError(f"CURVE_WALL must be 0 or 1, not {qi.curve_wall}")
+    if qi.curve_wall == 1 and p.restart != 1:
+        raise ValueError("CURVE_WALL = 1 needs restart = 1: the reference's restart = 0 start "
+                         "reads coordinates it leaves unset for 8-node cells")
     if p.restart == 1 and qi.restart_ord != p.N:
         # initsetup.f90:157-161 prints and leaves Q unset; refuse instead
         raise ValueError(f"restart_ord {qi.restart_ord} differs from N {p.N}")
     mesh = M.read_mesh_files(os.path.join(d, qi.namecel), os.path.join(d, qi.namevrt),
                              os.path.join(d, qi.namebnd), p.dxcycl, p.dycycl)
+    if qi.curve_wall == 1:
+        mesh.x8 = M.read_vrt8(os.path.join(d, qi.namevrt8), mesh.ncell)
     case = M.build_case(mesh, p)
     g = Sd2d(case, precision=precision)
     try:

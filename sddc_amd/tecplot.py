@@ -49,21 +49,25 @@ class PlotTopology:
         return self.xy.shape[1]
 
 
-def _xy_at(xv4, yv4, xsi, eta):
-    """xyCoor_atGps with the bilinear base functions (mapder.f90:79-87,
-    255-273): the four vertex terms summed in order from 0."""
-    f = ((1.0 - xsi) * (1.0 - eta), (1.0 - eta) * xsi, xsi * eta, (1.0 - xsi) * eta)
-    x = 0.0 * xv4[..., 0]
-    y = 0.0 * yv4[..., 0]
-    for k in range(4):
-        x = x + xv4[..., k] * f[k]
-        y = y + yv4[..., k] * f[k]
+def _xy_at(xn, yn, xsi, eta):
+    """xyCoor_atGps (mapder.f90:255-273) over the last axis of xn/yn: four
+    vertices with the bilinear base functions or eight nodes with the
+    8-node ones (mapder.f90:79-102), the node terms summed in order from 0."""
+    K = xn.shape[-1]
+    f = M.base_functions(xsi, eta, K)
+    x = 0.0 * xn[..., 0]
+    y = 0.0 * yn[..., 0]
+    for k in range(K):
+        x = x + xn[..., k] * f[k]
+        y = y + yn[..., k] * f[k]
     return x, y
 
 
 def plot_topology(case: M.Case, nre: int) -> PlotTopology:
-    """The plot mesh of tecplotter2dsetup.f90:32-152 for a mesh of bilinear
-    cells (CURVE_WALL = 0)."""
+    """The plot mesh of tecplotter2dsetup.f90:32-152: the cell vertices,
+    then the nodes of the refined faces and cell interiors, placed with the
+    four vertices (CURVE_WALL = 0) or with the eight nodes of the .vrt8
+    file (CURVE_WALL = 1, tecplotter2dsetup.f90:58-66)."""
     if nre < 1:
         raise ValueError("NRE must be at least 1")
     mesh = case.mesh
@@ -97,7 +101,11 @@ def plot_topology(case: M.Case, nre: int) -> PlotTopology:
         e = np.arange(1, Np)
         face_ij = {0: (e + 1, np.ones_like(e)), 1: (np.full_like(e, Np + 1), e + 1),
                    2: (Np + 1 - e, np.full_like(e, Np + 1)), 3: (np.ones_like(e), Np + 1 - e)}
-        X4, Y4 = xv[cells], yv[cells]                            # (nc, 4)
+        if case.params.curve_wall == 1:
+            X4 = np.ascontiguousarray(mesh.x8[:, :, 0])           # (nc, 8)
+            Y4 = np.ascontiguousarray(mesh.x8[:, :, 1])
+        else:
+            X4, Y4 = xv[cells], yv[cells]                        # (nc, 4)
         # the first (cell, k) visit of every face creates its nodes
         occ = ic2f.ravel()                                      # visit order cell*4 + k
         order = np.arange(occ.size)

@@ -1,10 +1,12 @@
 #!/usr/bin/env python3
-"""Capture the reference program's Tecplot output for the C1 case into
-tests/golden/c1_tecplot.npz: the unmodified program (oracle/_ref/sd2d_MHD,
-built by oracle/ref.mk) runs one step of the C1 deck with NRE = 1 and with
-NRE = 3 and writes TEC/*.plt at iteration 1 (iterations.f90:76-81); the
-fixture holds those files' bytes and names.  Runs only where the reference
-sources are.
+"""Capture the reference program's Tecplot output: the unmodified program
+(oracle/_ref/sd2d_MHD, built by oracle/ref.mk) runs one step of a deck and
+writes TEC/*.plt at iteration 1 (iterations.f90:76-81); each fixture holds
+those files' bytes and names.
+  tests/golden/c1_tecplot.npz              the C1 deck, NRE = 1 and 3
+  tests/golden/ogrid_curved_tecplot.npz    the 8-node O-grid of the
+                                           ogrid_curved4x8_N4 fixture, NRE = 3
+Runs only where the reference sources are.
 
     python tests/golden/make_tecplot_golden.py
 """
@@ -26,11 +28,14 @@ import refrun  # noqa: E402
 from specs import build  # noqa: E402
 
 
-def main() -> None:
-    refrun.build_reference()
+FIXTURES = (("c1_tecplot.npz", "c1_cylinder", (1, 3)),
+            ("ogrid_curved_tecplot.npz", "ogrid_curved4x8_N4", (3,)))
+
+
+def capture(spec: str, nres) -> dict:
     out = {}
-    for nre in (1, 3):
-        case, Q0 = build("c1_cylinder")
+    for nre in nres:
+        case, Q0 = build(spec)
         case.params.nre = nre
         d = tempfile.mkdtemp(prefix="sd2dtec_")
         try:
@@ -43,7 +48,13 @@ def main() -> None:
             print(f"NRE {nre}: {files[0]}, {out[f'nre{nre}_plt'].size} bytes")
         finally:
             shutil.rmtree(d, ignore_errors=True)
-    np.savez_compressed(os.path.join(HERE, "c1_tecplot.npz"), **out)
+    return out
+
+
+def main() -> None:
+    refrun.build_reference()
+    for fn, spec, nres in FIXTURES:
+        np.savez_compressed(os.path.join(HERE, fn), **capture(spec, nres))
 
 
 if __name__ == "__main__":

@@ -32,6 +32,16 @@ SPECS = {
                                dict(kind="smooth", amp=0.05, seed=3), 2, 2),
     "fsrk_ogrid4x8_N4": (dict(kind="ogrid", nr=4, nt=8, r0=0.5, r1=15.0),
                          dict(N=4, dt=1e-3, k_stage=5), dict(kind="smooth", amp=0.05, seed=4), 0, 3),
+    # 8-node cells (CURVE_WALL = 1, mapder.f90:91-253): an O-grid whose arc
+    # edges follow their circles through a midside node; the two cells of a
+    # face form its normal in different orders, so the normals differ by
+    # rounding and the device takes the left cell's (interfaceflux_all.f90:47)
+    "ogrid_curved4x8_N4": (dict(kind="ogrid", nr=4, nt=8, r0=0.5, r1=15.0, curved=True),
+                           dict(N=4, dt=1e-3, curve_wall=1),
+                           dict(kind="smooth", amp=0.05, seed=6), 2, 3),
+    "viscous_ogrid_curved3x10_N3": (dict(kind="ogrid", nr=3, nt=10, r0=0.5, r1=4.0, curved=True),
+                                    dict(N=3, dt=5e-4, mu=0.01, eta0=0.02, curve_wall=1),
+                                    dict(kind="smooth", amp=0.05, seed=7), 1, 2),
     "c1_cylinder": (dict(kind="ogrid", nr=32, nt=64, r0=0.5, r1=15.0),
                     dict(N=4, dt=5e-4),
                     dict(kind="freestream", uv_pert=0.02, seed=5, ramp=0.5), 1, 2),
@@ -42,7 +52,7 @@ def make_mesh(spec: dict) -> M.Mesh:
     s = dict(spec)
     kind = s.pop("kind")
     if kind == "ogrid":
-        return M.ogrid(s["nr"], s["nt"], s["r0"], s["r1"])
+        return M.ogrid(s["nr"], s["nt"], s["r0"], s["r1"], curved=s.get("curved", False))
     if kind == "channel":
         if "types" in s:
             s["types"] = tuple(s["types"])

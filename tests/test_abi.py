@@ -180,19 +180,35 @@ def test_unlisted_face_is_a_processor_interface():
     assert rc == abi.SD2D_EINVAL and "interface" in msg
 
 
-def test_nonconforming_normals_refused():
-    """The kernels take a face's left normal as +-(right cell's own metric);
-    a mesh where this does not hold bit for bit is refused."""
-    case = _small_case()
-
+def _perturbed_s1(case, rel):
     def mut(ms, keep):
         s1 = np.array(case.s1, dtype=np.float64, order="F").copy(order="F")
-        s1[0, 0, 0, 1, 0] *= 1.0 + 1e-9      # perturb one face-point metric of cell 1
+        s1[0, 0, 0, 1, 0] *= 1.0 + rel      # one face-point normal component of cell 1
         a = np.ascontiguousarray(s1.ravel(order="F"))
         keep.append(a)
         ms.s1 = a.ctypes.data_as(C.POINTER(C.c_double))
-    rc, _, msg = _create(case, mut)
+    return mut
+
+
+def test_nonconforming_normals_refused():
+    """The two cells of a face must see one normal up to sign: normals that
+    differ by more than rounding (1e-9 of their length) are refused, as a
+    non-conforming face."""
+    case = _small_case()
+    rc, _, msg = _create(case, _perturbed_s1(case, 1e-6))
     assert rc == abi.SD2D_EUNSUPPORTED and "non-conforming" in msg, msg
+
+
+def test_rounding_different_normals_accepted():
+    """Normals that differ by rounding (as the two cells of a curved 8-node
+    face form them) pass the face-table check: the right cell then takes
+    the left cell's normal.  Without a GPU the call stops later, at the
+    device, not at the check."""
+    case = _small_case()
+    rc, ctx, msg = _create(case, _perturbed_s1(case, 4e-16))
+    if rc == abi.SD2D_OK:
+        solver._lib.sd2d_destroy(ctx)
+    assert "non-conforming" not in msg and "normal" not in msg, msg
 
 
 def test_product_path_needs_the_library(tmp_path):

@@ -4,8 +4,10 @@ files).
 * The device's nodal values equal the oracle's restatement of tecplotter2d
   (tecplotter.f90:17-60) bit for bit on the same state, on O-grid, channel
   and periodic meshes, N = 3, 4, 5 and NRE = 1, 2, 3.
-* The run driver on the C1 deck (one step, NRE = 1 and 3) writes the file
-  the reference program wrote at that step (tests/golden/c1_tecplot.npz):
+* The run driver on the C1 deck (one step, NRE = 1 and 3) and on a deck of
+  8-node cells (CURVE_WALL = 1, NRE = 3) writes the file the reference
+  program wrote at that step (tests/golden/c1_tecplot.npz,
+  tests/golden/ogrid_curved_tecplot.npz):
   the same name, size, header, coordinates, AV fields and connectivity
   byte for byte; each of the 9 flow variables to one float32 spacing of
   the reference's value plus 1e-11 of that variable's largest magnitude,
@@ -31,7 +33,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 
 
 @pytest.mark.parametrize("name,nre", [("ogrid4x8_N4", 1), ("channel5x4_N3", 3),
-                                      ("periodic4x4_N5", 2), ("ogrid8x16_N3", 3)])
+                                      ("periodic4x4_N5", 2), ("ogrid8x16_N3", 3),
+                                      ("ogrid_curved5x12_N5", 3)])
 def test_plot_nodes_bitwise(name, nre):
     from oracle.tecplot_oracle import nodal_values
     from sddc_amd.solver import Sd2d
@@ -62,12 +65,14 @@ def _split(raw: bytes, nn: int, ncp: int):
     return head, mm, data, conn
 
 
-@pytest.mark.parametrize("nre", [1, 3])
-def test_run_driver_writes_reference_tecplot(tmp_path, nre):
+@pytest.mark.parametrize("fn,spec,nre", [("c1_tecplot.npz", "c1_cylinder", 1),
+                                         ("c1_tecplot.npz", "c1_cylinder", 3),
+                                         ("ogrid_curved_tecplot.npz", "ogrid_curved4x8_N4", 3)])
+def test_run_driver_writes_reference_tecplot(tmp_path, fn, spec, nre):
     from sddc_amd import run as R
-    print(f"[gpu-tecplot] run driver C1 NRE {nre}", flush=True)
-    fx = np.load(os.path.join(HERE, "golden", "c1_tecplot.npz"))
-    case, Q0 = specs.build("c1_cylinder")
+    print(f"[gpu-tecplot] run driver {spec} NRE {nre}", flush=True)
+    fx = np.load(os.path.join(HERE, "golden", fn))
+    case, Q0 = specs.build(spec)
     case.params.nre = nre
     refrun.write_case(str(tmp_path), case, Q0, 0.0, maxiter=1, nwrite=100000)
     out = R.run_dir(str(tmp_path))

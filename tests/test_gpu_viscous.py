@@ -27,15 +27,20 @@ def _gpu():
     return Sd2d
 
 
-@pytest.mark.parametrize("fixture", ["viscous_channel6x5_N4", "viscous_periodic5x4_N3"])
+@pytest.mark.parametrize("fixture", ["viscous_channel6x5_N4", "viscous_periodic5x4_N3",
+                                     "viscous_ogrid_curved3x10_N3", "ogrid_curved4x8_N4"])
 def test_viscous_fixture_against_reference(fixture):
     """Every staged residual evaluation and the state after all steps of the
-    reference's viscous runs: a channel with every boundary type and a
+    reference's viscous runs: a channel with every boundary type, a
     periodic square, whose BR1 gradient averages cross the CYCL pairs
-    (cyclocflux_all.f90:114-204)."""
+    (cyclocflux_all.f90:114-204), and an O-grid of 8-node cells with the
+    adiabatic wall on its curved inner edge; and of the reference's
+    inviscid run on 8-node cells (CURVE_WALL = 1), whose face normals the
+    device takes from the left cell as the reference does."""
     print(f"[gpu-viscous] fixture {fixture}", flush=True)
     case, Q0, fx = specs.load(fixture)
-    assert case.params.mu == MU and case.params.eta0 == ETA0
+    if fixture.startswith("viscous"):
+        assert case.params.mu == MU and case.params.eta0 == ETA0
     meta = json.loads(str(fx["meta"]))
     g = _gpu()(case)
     N, nc = case.N, case.ncell

@@ -100,16 +100,6 @@ def test_report_schedule():
     assert R._next(7, 1) > 10 ** 9      # mod(iter, 1) is never 1
 
 
-def test_run_refuses_curved_walls(tmp_path):
-    case, Q0 = cases.c1_cylinder()
-    M.write_quad_inp(str(tmp_path / "QUAD.INP"), case.params)
-    txt = (tmp_path / "QUAD.INP").read_text().splitlines()
-    txt[10] = "1"                        # CURVE_WALL
-    (tmp_path / "QUAD.INP").write_text("\n".join(txt) + "\n")
-    with pytest.raises(NotImplementedError):
-        R.run_dir(str(tmp_path))
-
-
 def _pairs_all(m, topo, dxc, dyc):
     """All-pairs form of the reference's midpoint test (map_cyc_loc.f90:15-35)."""
     cyc = topo.lists["CYCL"]

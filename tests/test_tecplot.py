@@ -1,9 +1,12 @@
 """The Tecplot output pipeline on the CPU (tecplotter2dsetup.f90,
 tecplotter.f90): the plot mesh of sddc_amd.tecplot, the nodal values of the
 oracle's restatement of tecplotter2d and the #!TDV111 writer, from the
-oracle's state after one step of the C1 deck (bit for bit the reference
+oracle's state after one step of a deck (bit for bit the reference
 program's state), against the file the reference program itself wrote at
-that step (tests/golden/c1_tecplot.npz, NRE = 1 and 3): byte for byte."""
+that step: byte for byte.  Decks: C1 (tests/golden/c1_tecplot.npz, NRE = 1
+and 3) and the 8-node O-grid of the ogrid_curved4x8_N4 fixture, whose plot
+nodes lie on the cells' quadratic edges
+(tests/golden/ogrid_curved_tecplot.npz, NRE = 3)."""
 import os
 
 import numpy as np
@@ -16,8 +19,8 @@ from tests.golden import specs
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 
-def _fixture():
-    return np.load(os.path.join(HERE, "golden", "c1_tecplot.npz"))
+def _fixture(fn="c1_tecplot.npz"):
+    return np.load(os.path.join(HERE, "golden", fn))
 
 
 @pytest.mark.parametrize("nre", [1, 3])
@@ -35,12 +38,14 @@ def test_plot_mesh_counts(nre):
     assert np.all(area > 0)
 
 
-@pytest.mark.parametrize("nre", [1, 3])
-def test_plt_matches_reference_program(tmp_path, nre):
+@pytest.mark.parametrize("fn,spec,nre", [("c1_tecplot.npz", "c1_cylinder", 1),
+                                         ("c1_tecplot.npz", "c1_cylinder", 3),
+                                         ("ogrid_curved_tecplot.npz", "ogrid_curved4x8_N4", 3)])
+def test_plt_matches_reference_program(tmp_path, fn, spec, nre):
     from oracle.oracle import Oracle
     from oracle.tecplot_oracle import nodal_values
-    fx = _fixture()
-    case, Q0 = specs.build("c1_cylinder")
+    fx = _fixture(fn)
+    case, Q0 = specs.build(spec)
     o = Oracle(case)
     o.set_state(Q0, [0, 0, 0, 0])
     o.set_time(0.0, 0)

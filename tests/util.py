@@ -34,6 +34,12 @@ CASES = {
     "ogrid2x193_N5": (lambda: M.ogrid(2, 193, 5.0, 6.0), dict(N=5, dt=5e-4), "smooth"),
     "periodic241x2_N4": (lambda: M.periodic_square(241, 2, 24.1, 0.2), dict(N=4, dt=1e-3), "smooth"),
     "ogrid2x321_N3": (lambda: M.ogrid(2, 321, 5.0, 5.6), dict(N=3, dt=5e-4), "smooth"),
+    # 8-node cells (CURVE_WALL = 1): the two cells of a face form its normal
+    # in different orders, so the device takes the left cell's
+    "ogrid_curved4x8_N4": (lambda: M.ogrid(4, 8, 0.5, 15.0, curved=True),
+                           dict(N=4, dt=1e-3, curve_wall=1), "smooth"),
+    "ogrid_curved5x12_N5": (lambda: M.ogrid(5, 12, 0.5, 6.0, curved=True),
+                            dict(N=5, dt=5e-4, curve_wall=1), "smooth"),
 }
 
 
