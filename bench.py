@@ -83,6 +83,57 @@ def mixed_to_time(case, Q0, t_end: float, desc: str) -> dict:
             "all_finite": bool(np.all(np.isfinite(a)))}
 
 
+def rounding_sensitivity(case, Q0, checkpoints, desc: str) -> dict:
+    """How far rounding-level changes carry on this workload: four runs
+    stepped side by side,
+      fp64            FP64 from Q0,
+      fp64_q0_f32     FP64 from Q0 rounded once to float32,
+      fp64_q0_1e-15   FP64 from Q0 * (1 + 1e-15 * seeded U[-1, 1]),
+      mixed           the mixed form from Q0,
+    and at every checkpoint step the per-variable L2 difference of the last
+    three from the first, against each variable's own norm and against the
+    norm of rho*u (the momentum scale of the flow)."""
+    import numpy as np
+    from sddc_amd import abi
+    from sddc_amd.solver import Sd2d
+    rng = np.random.default_rng(7)
+    starts = {"fp64": (Q0, abi.SD2D_FP64),
+              "fp64_q0_f32": (Q0.astype(np.float32).astype(np.float64), abi.SD2D_FP64),
+              "fp64_q0_1e-15": (Q0 * (1.0 + 1e-15 * rng.uniform(-1, 1, Q0.shape)), abi.SD2D_FP64),
+              "mixed": (Q0, abi.SD2D_MIXED)}
+    runs = {}
+    for name, (q, prec) in starts.items():
+        g = Sd2d(case, precision=prec)
+        g.set_state(q, [0.0, 0.0, 0.0, 0.0])
+        runs[name] = g
+    rows, done = [], 0
+    try:
+        for k in checkpoints:
+            for g in runs.values():
+                g.step(k - done, 0)
+            done = k
+            st = {n: g.get_state()[0].reshape(9, -1, order="F") for n, g in runs.items()}
+            a = st["fp64"]
+            na = np.sqrt((a * a).sum(axis=1))
+            row = {"step": k, "norm": [float(x) for x in na]}
+            for n in ("fp64_q0_f32", "fp64_q0_1e-15", "mixed"):
+                nd = np.sqrt(((st[n] - a) ** 2).sum(axis=1))
+                row[n + "_vs_own"] = [float(x) for x in nd / np.where(na > 0, na, 1.0)]
+                row[n + "_vs_rhou"] = [float(x) for x in nd / na[1]]
+            rows.append(row)
+            print(f"[bench] sensitivity step {k}: rho*v vs own norm  f32 start "
+                  f"{row['fp64_q0_f32_vs_own'][2]:.2e}  1e-15 start "
+                  f"{row['fp64_q0_1e-15_vs_own'][2]:.2e}  mixed {row['mixed_vs_own'][2]:.2e}",
+                  file=sys.stderr, flush=True)
+    finally:
+        for g in runs.values():
+            g.close()
+    return {"workload": desc, "dt": case.params.dt, "checkpoints": rows,
+            "definition": "||q_run - q_fp64||_2 per variable over all points, divided by "
+                          "the FP64 run's own norm of that variable (vs_own) or by its "
+                          "norm of rho*u (vs_rhou)"}
+
+
 def run_c4(steps: int, warmup: int, tune_steps: int = 2) -> dict:
     """The C4 ensemble (64 cylinder runs of uneven size and order) stepped
     as one sd2d_ens ensemble in mixed precision after the tile-shape
@@ -268,6 +319,10 @@ def main() -> int:
     ap.add_argument("--t-end", type=float, default=None,
                     help="instead of the throughput run: FP64 and mixed runs to this "
                          "time, reporting the mixed form's L2 difference")
+    ap.add_argument("--sensitivity", default=None, metavar="STEPS",
+                    help="instead of the throughput run: FP64 from the workload's state, "
+                         "from it rounded to float32 and from it perturbed by 1e-15, and "
+                         "the mixed form, compared at these comma-separated steps")
     ap.add_argument("--cpu-steps", type=int, default=3,
                     help="steps of the CPU baseline sample (0 = skip); 3 C2 steps are "
                          "about 10 s of the oracle on 16 host threads")
@@ -302,6 +357,13 @@ def main() -> int:
           f"built in {time.perf_counter() - t:.1f}s", file=sys.stderr, flush=True)
     if args.t_end is not None:
         print(json.dumps(mixed_to_time(case, Q0, args.t_end, desc)), flush=True)
+        return 0
+    if args.sensitivity is not None:
+        ck = sorted({int(x) for x in args.sensitivity.split(",") if x.strip()})
+        if not ck or ck[0] < 1:
+            sys.stderr.write("--sensitivity takes positive step numbers\n")
+            return 2
+        print(json.dumps(rounding_sensitivity(case, Q0, ck, desc)), flush=True)
         return 0
     N = case.N
     ndof = cases.dof(case)
