@@ -11,12 +11,16 @@
 //          traces and rounded after the cancellation), the Jacobian scaling
 //          of the residual, the Powell and GLM sources, the RK stage
 //          combination, the norms and the AV sensor.
-// The tile's state is staged in LDS as FP32 for the traces; the points step
-// reads the FP64 state from memory (one point ahead of its use), and the
-// FP32 staging area is reused for the exchange of the derivatives between
-// the two line families, which split the points' work by variable.
-// About 1.1 KB of LDS per cell and roughly half the registers of the FP64
-// kernel, so more workgroups share a CU.
+// The tile's state is staged in LDS as FP32 for the traces and stays there
+// through the points step, whose Powell source reads the velocity and the
+// field from it (FP64 arithmetic on the FP32-rounded values, the precision
+// of the FP32 divB they multiply); from memory the points step reads only
+// the FP64 values of the variables the thread updates, for the RK
+// combination (one point ahead of their use).  The derivatives are handed
+// between the two line families, which split the points' work by
+// variable, through an area of their own.  About 1.6 KB of LDS per cell and
+// roughly half the registers of the FP64 kernel, so more workgroups share
+// a CU.
 #pragma once
 #include "sd2d_stage.hpp"
 
@@ -31,10 +35,13 @@ struct OpsF {
 template <int N, int MODE> struct StageLdsMx {
   static constexpr int T = Geo<N>::CPW, N2 = N * N, NL = N - 1;
   static constexpr bool AV = (MODE & M_AV) != 0;
-  // in floats: the staged state [v][p][ci], reused for the eta-line flux
-  // derivatives; then the divB and Jz derivatives [p][ci]
+  // in floats: the staged state [v][p][ci]; the flux derivatives of up to
+  // XV variables [v][p][ci] handed from one family to the other (eta -> xi:
+  // variables 0-4, then xi -> eta: 5-8); the divB and Jz derivatives [p][ci]
+  static constexpr int XV = 5;
   static constexpr int QF = 0;
-  static constexpr int XB = 9 * N2 * T;
+  static constexpr int XD = 9 * N2 * T;
+  static constexpr int XB = XD + XV * N2 * T;
   static constexpr int XJ = XB + N2 * T;
   static constexpr int FLOATS = XJ + (AV ? N2 * T : 0);
   // in doubles, after the float area: reduction slots, and the AV sensor
@@ -220,15 +227,13 @@ void stage_kernel_mx(StageArgs a, OpsF ops, const float *__restrict__ metf) {
 
   // ---- the two families share the solution points' work by variable, as in
   //      stage_kernel: xi-lines finish variables 0..VS-1, eta-lines VS..8.
-  //      The FP64 state and RK register of a thread's first point are loaded
-  //      now; every later point one point ahead of its use, before the
-  //      stores of the point in hand ------------------------------------------------
-  constexpr int VS = 5;
+  //      The FP64 values of a thread's own variables and its RK register at
+  //      its first point are loaded now; every later point's one point ahead
+  //      of their use, before the stores of the point in hand ----------------------
+  constexpr int VS = LD::XV;
   const double *qg = a.qin + (size_t)tile * G::QT + ci;
   const double *aux0 = (NAUX == 1 ? a.aux[0] : a.qin) + (size_t)tile * G::QT + ci;
-  // the velocity and the field enter the sources of every variable; the
-  // energy and psi only their own update
-  auto needed = [](int v, int v0, int v1) { return v < 4 || (v >= 5 && v < 8) || (v >= v0 && v < v1); };
+  auto needed = [](int v, int v0, int v1) { return v >= v0 && v < v1; };
   double qn[9], xn[VS], jis[N];
   if (valid) {
 #pragma unroll
@@ -244,17 +249,16 @@ void stage_kernel_mx(StageArgs a, OpsF ops, const float *__restrict__ metf) {
     for (int v = 0; v < VS; ++v)
       xn[v] = (NAUX == 1 && v0 + v < v1) ? aux0[((v0 + v) * N2 + p0) * T] : 0.0;
   }
-  __syncthreads();   // every thread is done with the staged state
-
-  // ---- exchange through the former staging area (one slot per variable):
+  // ---- exchange through the derivative area (one slot per variable):
   //      round A, eta -> xi: variables 0..VS-1 and the divB (Jz) weights;
-  //      round B, xi -> eta: variables VS..8 and the summed divB weight ---------
+  //      round B, xi -> eta: variables VS..8 and the summed divB weight.  The
+  //      staged state stays as it is for the sources of the points step ----
   if (valid && fam == 1) {
 #pragma unroll
     for (int s = 0; s < N; ++s) {
       const int p = p0 + ps * s;
 #pragma unroll
-      for (int v = 0; v < VS; ++v) ldf[LD::QF + (v * N2 + p) * T + ci] = D[s][v];
+      for (int v = 0; v < VS; ++v) ldf[LD::XD + (v * N2 + p) * T + ci] = D[s][v];
       ldf[LD::XB + p * T + ci] = db[s];
       if (AV) ldf[LD::XJ + p * T + ci] = dj[s];
     }
@@ -265,11 +269,13 @@ void stage_kernel_mx(StageArgs a, OpsF ops, const float *__restrict__ metf) {
     for (int s = 0; s < N; ++s) {
       const int p = N * line + s;
 #pragma unroll
-      for (int v = 0; v < VS; ++v) D[s][v] += ldf[LD::QF + (v * N2 + p) * T + ci];
+      for (int v = 0; v < VS; ++v) D[s][v] += ldf[LD::XD + (v * N2 + p) * T + ci];
       db[s] += ldf[LD::XB + p * T + ci];
       if (AV) dj[s] += ldf[LD::XJ + p * T + ci];
+      // round B into the slots of this point just read (only this thread
+      // reads or writes the slots of its points)
 #pragma unroll
-      for (int v = VS; v < 9; ++v) ldf[LD::QF + (v * N2 + p) * T + ci] = D[s][v];
+      for (int v = VS; v < 9; ++v) ldf[LD::XD + ((v - VS) * N2 + p) * T + ci] = D[s][v];
       ldf[LD::XB + p * T + ci] = db[s];
     }
   }
@@ -279,7 +285,7 @@ void stage_kernel_mx(StageArgs a, OpsF ops, const float *__restrict__ metf) {
     for (int s = 0; s < N; ++s) {
       const int p = p0 + ps * s;
 #pragma unroll
-      for (int v = VS; v < 9; ++v) D[s][v] = ldf[LD::QF + (v * N2 + p) * T + ci] + D[s][v];
+      for (int v = VS; v < 9; ++v) D[s][v] = ldf[LD::XD + ((v - VS) * N2 + p) * T + ci] + D[s][v];
       db[s] = ldf[LD::XB + p * T + ci];
     }
   }
@@ -297,6 +303,10 @@ void stage_kernel_mx(StageArgs a, OpsF ops, const float *__restrict__ metf) {
 #pragma unroll
       for (int v = 0; v < 9; ++v)
         if (needed(v, V0, V1)) q[v] = qn[v];
+      // the velocity and the field of the sources, from the staged state
+      const float *st = ldf + LD::QF + p * T + ci;
+      const double w0 = st[0], w1 = st[N2 * T], w2 = st[2 * N2 * T], w3 = st[3 * N2 * T];
+      const double w5 = st[5 * N2 * T], w6 = st[6 * N2 * T], w7 = st[7 * N2 * T];
 #pragma unroll
       for (int v = 0; v < V1 - V0; ++v) xc[v] = xn[v];
 #pragma unroll
@@ -311,13 +321,13 @@ void stage_kernel_mx(StageArgs a, OpsF ops, const float *__restrict__ metf) {
           for (int v = V0; v < V1; ++v) xn[v - V0] = aux0[(v * N2 + p + ps) * T];
         }
       }
-      const double ri = 1.0 / q[0];
-      const double ux = q[1] * ri, uy = q[2] * ri, uz = q[3] * ri;
+      const double ri = 1.0 / w0;
+      const double ux = w1 * ri, uy = w2 * ri, uz = w3 * ri;
       // Powell source and GLM damping, compresid.f90:58-74
-      if (V0 <= 1 && 1 < V1) R[1] -= divB * q[5];
-      if (V0 <= 2 && 2 < V1) R[2] -= divB * q[6];
-      if (V0 <= 3 && 3 < V1) R[3] -= divB * q[7];
-      if (V0 <= 4 && 4 < V1) R[4] -= divB * (ux * q[5] + uy * q[6] + uz * q[7]);
+      if (V0 <= 1 && 1 < V1) R[1] -= divB * w5;
+      if (V0 <= 2 && 2 < V1) R[2] -= divB * w6;
+      if (V0 <= 3 && 3 < V1) R[3] -= divB * w7;
+      if (V0 <= 4 && 4 < V1) R[4] -= divB * (ux * w5 + uy * w6 + uz * w7);
       if (V0 <= 5 && 5 < V1) R[5] -= divB * ux;
       if (V0 <= 6 && 6 < V1) R[6] -= divB * uy;
       if (V0 <= 7 && 7 < V1) R[7] -= divB * uz;
