@@ -166,7 +166,9 @@ def run_c4(steps: int, warmup: int, tune_steps: int = 2) -> dict:
         res[prec] = {"value": ndof * 5 * steps / wall, "wall_s": wall,
                      "finite_members": int(sum(fin)),
                      "value_finite_members": dof_fin * 5 * steps / wall,
-                     "tile_shapes": {str(n): ("strips" if v == abi.SD2D_TILE_STRIPS else "blocks")
+                     "tile_shapes": {str(n): {abi.SD2D_TILE_STRIPS: "strips",
+                                              abi.SD2D_TILE_BLOCKS: "blocks",
+                                              abi.SD2D_TILE_ROWPAIRS: "rowpairs"}.get(v, str(v))
                                      for n, v in shapes.items()},
                      "tune_s": tt}
         print(f"[bench] c4 precision {prec}: {steps} steps in {wall:.2f}s, shapes {shapes}",
@@ -331,8 +333,9 @@ def main() -> int:
     ap.add_argument("--cpu-ref-ranks", type=int, default=0,
                     help="MPI ranks of the reference program (0: this host's CPU share, "
                          "OMP_NUM_THREADS or the core count, at most 16)")
-    ap.add_argument("--tile-shape", default="strips", choices=["strips", "blocks"],
-                    help="cells of one workgroup: mesh-order strips (default) or compact blocks")
+    ap.add_argument("--tile-shape", default="rowpairs", choices=["strips", "blocks", "rowpairs"],
+                    help="cells of one workgroup: two mesh rows (default; strips where the "
+                         "mesh is not numbered row by row), mesh-order strips, or compact blocks")
     ap.add_argument("--no-mixed", action="store_true",
                     help="skip the mixed-precision comparison")
     args = ap.parse_args()
@@ -367,7 +370,8 @@ def main() -> int:
         return 0
     N = case.N
     ndof = cases.dof(case)
-    shape = abi.SD2D_TILE_STRIPS if args.tile_shape == "strips" else abi.SD2D_TILE_BLOCKS
+    shape = {"strips": abi.SD2D_TILE_STRIPS, "blocks": abi.SD2D_TILE_BLOCKS,
+             "rowpairs": abi.SD2D_TILE_ROWPAIRS}[args.tile_shape]
     g = Sd2d(case, tile_shape=shape)
     g.set_state(Q0, [0.0, 0.0, 0.0, 0.0])
     g.step(args.warmup, 10)

@@ -55,12 +55,17 @@ enum { SD2D_FP64 = 0, SD2D_MIXED = 1 };
  * consecutive rows line up); SD2D_TILE_BLOCKS groups compact patches of cells
  * (recursive bisection of the cell adjacency graph), so fewer neighbour
  * traces come from other tiles, but those that do are scattered over more
- * tiles.  SD2D_TILE_DEFAULT is SD2D_TILE_STRIPS (the faster of the two on
- * structured O-grid and periodic meshes numbered row by row).  Results do
- * not depend on the shape (every face is evaluated with the same operands
- * either way); only the speed does.  No counterpart in the reference,
- * whose cells are processed in file order. */
-enum { SD2D_TILE_DEFAULT = 0, SD2D_TILE_STRIPS = 1, SD2D_TILE_BLOCKS = 2 };
+ * tiles.  SD2D_TILE_ROWPAIRS takes, on a mesh numbered row by row, two
+ * consecutive rows of 6/8/10 cells each, so that the faces between the two
+ * rows lie inside the tile and every line across the rows gathers at most
+ * one end from another tile (a last odd row is cut into strips; a mesh not
+ * numbered row by row gets strips).  SD2D_TILE_DEFAULT is
+ * SD2D_TILE_ROWPAIRS (measured fastest on structured O-grid and periodic
+ * meshes: 1-2 % over strips, which beat blocks).  Results do not depend on
+ * the shape (every face is evaluated with the same operands either way);
+ * only the speed does.  No counterpart in the reference, whose cells are
+ * processed in file order. */
+enum { SD2D_TILE_DEFAULT = 0, SD2D_TILE_STRIPS = 1, SD2D_TILE_BLOCKS = 2, SD2D_TILE_ROWPAIRS = 3 };
 
 /*
  * Mesh, operators and metrics exactly as module setup2d holds them after
@@ -332,7 +337,8 @@ int sd2d_set_check_finite(sd2d_ctx *ctx, int32_t every);
  *                         sd2d_set_time / sd2d_get_diag's ctime and iter
  *   sd2d_ens_step         nsteps RK_S5P3 steps of every member
  *   sd2d_ens_tune         autotuner: for each order, steps every tile shape
- *                         (SD2D_TILE_STRIPS, SD2D_TILE_BLOCKS) trial_steps
+ *                         (SD2D_TILE_ROWPAIRS, SD2D_TILE_STRIPS,
+ *                         SD2D_TILE_BLOCKS) trial_steps
  *                         times from the current states, keeps the fastest
  *                         and restores the states; shapes[N-3] returns the
  *                         choice for each order present (others untouched)

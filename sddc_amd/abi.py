@@ -16,7 +16,7 @@ from . import mesh as M
 SD2D_OK, SD2D_EINVAL, SD2D_EHIP, SD2D_ENOMEM, SD2D_ENONFINITE, \
     SD2D_EUNSUPPORTED = range(6)
 SD2D_FP64, SD2D_MIXED = 0, 1
-SD2D_TILE_DEFAULT, SD2D_TILE_STRIPS, SD2D_TILE_BLOCKS = 0, 1, 2
+SD2D_TILE_DEFAULT, SD2D_TILE_STRIPS, SD2D_TILE_BLOCKS, SD2D_TILE_ROWPAIRS = 0, 1, 2, 3
 # doubles of the static scratch of sd2d_compresid_noalloc (64 MiB)
 SD2D_NOALLOC_MAX_DOUBLES = 8 * 1024 * 1024
 

@@ -280,7 +280,7 @@ int sd2d_ens_create(int32_t nmem, const sd2d_mesh *meshes, const sd2d_params *pa
                       w + "precision, ARTIV and vismode must be the same for all members");
     if (p.precision != SD2D_FP64 && p.precision != SD2D_MIXED)
       return ens_fail(nullptr, SD2D_EINVAL, w + "precision must be SD2D_FP64 or SD2D_MIXED");
-    if (p.tile_shape < SD2D_TILE_DEFAULT || p.tile_shape > SD2D_TILE_BLOCKS)
+    if (p.tile_shape < SD2D_TILE_DEFAULT || p.tile_shape > SD2D_TILE_ROWPAIRS)
       return ens_fail(nullptr, SD2D_EINVAL, w + "bad tile_shape");
   }
   sd2d_ens *e = new (std::nothrow) sd2d_ens();
@@ -293,7 +293,7 @@ int sd2d_ens_create(int32_t nmem, const sd2d_mesh *meshes, const sd2d_params *pa
     for (int i = 0; i < nmem; ++i)
       if (meshes[i].N == n) { e->grp_n.push_back(n); break; }
   e->grp.assign(e->grp_n.size(), nullptr);
-  e->grp_shape.assign(e->grp_n.size(), SD2D_TILE_STRIPS);
+  e->grp_shape.assign(e->grp_n.size(), SD2D_TILE_ROWPAIRS);
   for (int i = 0; i < nmem; ++i)
     for (size_t g = 0; g < e->grp_n.size(); ++g)
       if (e->grp_n[g] == meshes[i].N) e->mem[i].group = (int)g;
@@ -310,10 +310,10 @@ int sd2d_ens_create(int32_t nmem, const sd2d_mesh *meshes, const sd2d_params *pa
       return bail(SD2D_EHIP);
     }
   for (size_t g = 0; g < e->grp_n.size(); ++g) {
-    int shape = SD2D_TILE_STRIPS;   // the group's shape: the first member's request
+    int shape = SD2D_TILE_ROWPAIRS;   // the group's shape: the first member's request
     for (int i = 0; i < nmem; ++i)
       if (e->mem[i].group == (int)g) {
-        shape = params[i].tile_shape == SD2D_TILE_BLOCKS ? SD2D_TILE_BLOCKS : SD2D_TILE_STRIPS;
+        shape = params[i].tile_shape == SD2D_TILE_DEFAULT ? SD2D_TILE_ROWPAIRS : params[i].tile_shape;
         break;
       }
     const int r = ens_build_group(e, (int)g, shape);
@@ -476,9 +476,9 @@ int sd2d_ens_tune(sd2d_ens *e, int32_t trial_steps, int32_t *shapes) {
   for (size_t g = 0; g < e->grp.size(); ++g) {
     int r = save((int)g);
     if (r) return r;
-    const int cand[2] = {SD2D_TILE_STRIPS, SD2D_TILE_BLOCKS};
+    const int cand[3] = {SD2D_TILE_ROWPAIRS, SD2D_TILE_STRIPS, SD2D_TILE_BLOCKS};
     double best = 1e300;
-    int best_shape = SD2D_TILE_STRIPS;
+    int best_shape = SD2D_TILE_ROWPAIRS;
     for (int shape : cand) {
       r = ens_build_group(e, (int)g, shape);
       if (r) return r;

@@ -280,6 +280,40 @@ inline void tile_slots(const sd2d_mesh *m, int CPW, int shape, std::vector<int> 
   std::vector<int> order(nc);
   for (int c = 0; c < nc; ++c) order[c] = c;
   c2s.assign(nc, 0);
+  if (shape == SD2D_TILE_ROWPAIRS) {
+    // two consecutive rows of CPW/2 cells each: the faces between the two
+    // rows lie inside the tile, so every line across the rows has at most
+    // one end in another tile.  A last odd row is cut into strips of CPW.
+    const int R = mesh_row_length(nc, pairs), H = CPW / 2;
+    if (R >= H) {
+      const int nrow = nc / R, nch = (R + H - 1) / H, nfull = (R + CPW - 1) / CPW;
+      const int nt = (nrow / 2) * nch + (nrow % 2) * nfull;
+      s2c.assign((size_t)nt * CPW, -1);
+      int t = 0;
+      for (int k = 0; k + 1 < nrow; k += 2)
+        for (int j = 0; j < nch; ++j, ++t) {
+          const int c0 = j * H, w = std::min(H, R - c0);
+          for (int i = 0; i < w; ++i) {
+            const int a = k * R + c0 + i, b = (k + 1) * R + c0 + i;
+            c2s[a] = t * CPW + i;
+            c2s[b] = t * CPW + w + i;
+            s2c[c2s[a]] = a;
+            s2c[c2s[b]] = b;
+          }
+        }
+      if (nrow % 2)
+        for (int j = 0; j < nfull; ++j, ++t) {
+          const int c0 = j * CPW, w = std::min(CPW, R - c0);
+          for (int i = 0; i < w; ++i) {
+            const int a = (nrow - 1) * R + c0 + i;
+            c2s[a] = t * CPW + i;
+            s2c[c2s[a]] = a;
+          }
+        }
+      return;
+    }
+    shape = SD2D_TILE_STRIPS;   // not numbered row by row
+  }
   if (shape == SD2D_TILE_BLOCKS) {
     std::vector<std::vector<int>> adj(nc);
     for (const auto &pr : pairs) {
@@ -361,11 +395,12 @@ inline int build_host_tables(const sd2d_mesh *m, const sd2d_params *p, HostTable
   t.ph.vinf = p->vinf; t.ph.bxinf = p->bxinf; t.ph.byinf = p->byinf;
   t.ph.vismode = p->vismode;
 
-  if (p->tile_shape < SD2D_TILE_DEFAULT || p->tile_shape > SD2D_TILE_BLOCKS) {
-    err = "tile_shape must be SD2D_TILE_DEFAULT, SD2D_TILE_STRIPS or SD2D_TILE_BLOCKS";
+  if (p->tile_shape < SD2D_TILE_DEFAULT || p->tile_shape > SD2D_TILE_ROWPAIRS) {
+    err = "tile_shape must be SD2D_TILE_DEFAULT, SD2D_TILE_STRIPS, SD2D_TILE_BLOCKS or "
+          "SD2D_TILE_ROWPAIRS";
     return SD2D_EINVAL;
   }
-  tile_slots(m, CPW, p->tile_shape == SD2D_TILE_BLOCKS ? SD2D_TILE_BLOCKS : SD2D_TILE_STRIPS,
+  tile_slots(m, CPW, p->tile_shape == SD2D_TILE_DEFAULT ? SD2D_TILE_ROWPAIRS : p->tile_shape,
              t.c2s, t.s2c);
   t.ntile = (int)(t.s2c.size() / CPW);
   std::vector<int> nb, info;

@@ -15,7 +15,7 @@ module sd2d_c
        SD2D_EUNSUPPORTED = 5
   integer(c_int32_t), parameter, public :: SD2D_FP64 = 0, SD2D_MIXED = 1
   integer(c_int32_t), parameter, public :: SD2D_TILE_DEFAULT = 0, &
-       SD2D_TILE_STRIPS = 1, SD2D_TILE_BLOCKS = 2
+       SD2D_TILE_STRIPS = 1, SD2D_TILE_BLOCKS = 2, SD2D_TILE_ROWPAIRS = 3
 
   type, bind(C), public :: sd2d_mesh_t
      integer(c_int32_t) :: N, ncell, nface

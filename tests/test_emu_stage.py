@@ -16,8 +16,8 @@ then lagged) to 1e-12 per variable, three RK_S5P3 / FiveStageRk steps to
 1e-11 on the increment, the mixed-precision kernel's steps to 1e-6.
 
 Meshes: every face type, N = 3, 4, 5, rows padded to the tile so padding
-slots lie inside the mesh, periodic pairs, 8-node curved cells, and both
-tile shapes."""
+slots lie inside the mesh, periodic pairs, 8-node curved cells, and the
+three tile shapes (strips, blocks, row pairs)."""
 import hashlib
 import os
 import shutil
@@ -144,6 +144,8 @@ def test_emulated_compresid(emu_exe, tmp_path, name):
                                                      ("channel7x6_N5", 3, 2), ("ogrid2x23_N5", 5, 0),
                                                      ("periodic31x2_N4", 3, 0),
                                                      ("periodic31x2_N4", 3, 2),
+                                                     ("periodic31x2_N4", 3, 3),
+                                                     ("ogrid2x23_N5", 3, 3),
                                                      ("ogrid_curved4x8_N4", 3, 0)])
 def test_emulated_rk_steps(emu_exe, tmp_path, name, k_stage, tile_shape):
     case, Q0 = _case(name, k_stage=k_stage)

@@ -63,7 +63,7 @@ def test_ensemble_tune_keeps_states():
     for i, (c, Q0) in enumerate(mem):
         ens.set_state(i, Q0, [0, 0, 0, 0])
     shapes = ens.tune(2)
-    assert set(shapes) == {3, 4, 5} and set(shapes.values()) <= {1, 2}, shapes
+    assert set(shapes) == {3, 4, 5} and set(shapes.values()) <= {1, 2, 3}, shapes
     ens.step(2)
     for i, (c, Q0) in enumerate(mem):
         Qe, mxe = ens.get_state(i)

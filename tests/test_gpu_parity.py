@@ -137,12 +137,13 @@ def test_tile_shapes_bitwise_equal(name):
     """The tile shape only decides which cells share a workgroup: the
     residual, the AV sensor, the state after whole steps and the maxima are
     bitwise the same for mesh-order strips and compact blocks (a face point
-    is evaluated from the same operands in the same roles either way)."""
+    is evaluated from the same operands in the same roles either way), and
+    for row pairs (two rows of the mesh per workgroup)."""
     print(f"[gpu-parity] tile shapes {name}", flush=True)
     from sddc_amd import abi
     case, Q0 = make_case(name)
     res = []
-    for shape in (abi.SD2D_TILE_STRIPS, abi.SD2D_TILE_BLOCKS):
+    for shape in (abi.SD2D_TILE_STRIPS, abi.SD2D_TILE_BLOCKS, abi.SD2D_TILE_ROWPAIRS):
         g = _gpu()(case, tile_shape=shape)
         g.set_state(Q0, [0, 0, 0, 0])
         r, nq = g.compresid()
@@ -152,5 +153,6 @@ def test_tile_shapes_bitwise_equal(name):
         Q, mx = g.get_state()
         res.append((r, nq, sm, Q, mx))
         g.close()
-    for a, b in zip(res[0], res[1]):
-        assert np.array_equal(np.asarray(a), np.asarray(b), equal_nan=True)
+    for other in res[1:]:
+        for a, b in zip(res[0], other):
+            assert np.array_equal(np.asarray(a), np.asarray(b), equal_nan=True)
