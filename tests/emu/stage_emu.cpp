@@ -19,7 +19,7 @@
 //               CYC2FPAIR(NPAIR,2)
 //   float64     Lmat, Mmat, Xs, Xs_low, Jac, S1, S2, Q(9,N,N,NCELL)
 // job 0: two sd2d_compresid evaluations (the second with the lagged c_h);
-//        OUT: resid, nablaQs, gmax(2) of each.
+//        OUT: resid, nablaQs, gmax(2), SM_IND(2,NCELL) of each.
 // job 1: nsteps RK steps (k_stage); OUT: Q(9,N,N,NCELL), gmax(2).
 #include <chrono>
 #include <condition_variable>
@@ -387,6 +387,7 @@ int main(int argc, char **argv) {
       put(c.resid_f.data(), n);
       put(c.nabla_f.data(), 2 * n);
       put(c.gmax.data(), 2);
+      put(c.sm_ind.data(), 2 * (size_t)nc);
     }
   } else {
     const int av = p.artiv == 1 ? M_AV : 0;

@@ -123,11 +123,12 @@ def test_emulated_compresid(emu_exe, tmp_path, name):
     case, Q0 = _case(name)
     n = 9 * case.N ** 2 * case.ncell
     out = _run(emu_exe, tmp_path, case, Q0, job=0)
-    assert out.size == 2 * (3 * n + 2)
+    nb = 3 * n + 2 + 2 * case.ncell
+    assert out.size == 2 * nb
     o = Oracle(case)
     o.set_state(Q0, [0, 0, 0, 0])
     for call in range(2):
-        blk = out[call * (3 * n + 2):(call + 1) * (3 * n + 2)]
+        blk = out[call * nb:(call + 1) * nb]
         ro, no = o.compresid()
         r = blk[:n].reshape(ro.shape, order="F")
         g = blk[n:3 * n].reshape(no.shape, order="F")
@@ -136,7 +137,15 @@ def test_emulated_compresid(emu_exe, tmp_path, name):
         e2, _ = per_var_error(g.reshape(18, -1, order="F"), no.reshape(18, -1, order="F"))
         assert e2.max() <= 1e-12, (name, call, e2)
         _, mo = o.get_state()
-        np.testing.assert_allclose(blk[3 * n:], np.asarray(mo)[2:4], rtol=1e-14, atol=0)
+        np.testing.assert_allclose(blk[3 * n:3 * n + 2], np.asarray(mo)[2:4], rtol=1e-14, atol=0)
+        # the AV smoothness sensor, to the device test's bar (log10 of a
+        # squared small difference over a squared value: 1e-15 in its
+        # inputs moves it by up to ~1e-8)
+        sm = blk[3 * n + 2:].reshape(2, -1, order="F")
+        so = np.asarray(o.get_av()[0]).reshape(2, -1, order="F")
+        fin = np.isfinite(so)
+        assert np.array_equal(np.isfinite(sm), fin), (name, call)
+        assert np.abs(sm[fin] - so[fin]).max() <= 1e-6, (name, call)
     o.close()
 
 
