@@ -339,7 +339,7 @@ static hipError_t launch_stage_mx_t(sd2d_ctx *c, const StageArgs &a) {
 
 template <int N, int MODE>
 static hipError_t launch_stage_t(sd2d_ctx *c, const StageArgs &a) {
-  if constexpr ((MODE & (M_COMPAT | M_VIS)) == 0) {
+  if constexpr ((MODE & (M_COMPAT | M_VIS | M_DIVB)) == 0) {
     if (c->p.precision == SD2D_MIXED) return launch_stage_mx_t<N, MODE>(c, a);
   }
   constexpr size_t lds = stage_lds_bytes<N, MODE>();
@@ -368,6 +368,7 @@ static hipError_t launch_stage_n(sd2d_ctx *c, const StageArgs &a, int mode) {
     SD2D_MODE(R | A4 | V | NR)
     SD2D_MODE(M_COMPAT)
     SD2D_MODE(M_COMPAT | V)
+    SD2D_MODE(M_DIVB)
     // the same with the viscous divergence (FP64 only)
     SD2D_MODE(M_VIS | R)
     SD2D_MODE(M_VIS | R | A1)

@@ -36,7 +36,7 @@ struct AdjArgs {
   const double *q;         // state the residual is taken of (tile layout)
   const double *rbar;      // cotangent of the residual (tile layout)
   double rscale;           // the cotangent used is rscale * rbar
-  const double *nabla_f;   // nablaQs of q, Fortran layout (for divB)
+  const double *divb;      // divB of q per solution point [tile][p][ci]
   const double *met;
   const int *ftab;
   const int *s2c;
@@ -101,8 +101,7 @@ __global__ __launch_bounds__(STAGE_THREADS) void adj_point_kernel(AdjArgs a) {
       q[v] = a.q[off + (size_t)(v * N2 + p) * T];
       rb[v] = a.rscale * a.rbar[off + (size_t)(v * N2 + p) * T];
     }
-    const double *nf = a.nabla_f + 18 * ((size_t)p + (size_t)N2 * cell);
-    const double divB = nf[5] + nf[9 + 6];
+    const double divB = a.divb[(size_t)tile * N2 * T + (size_t)p * T + ci];
     const double ri = 1.0 / q[0];
     const double ux = q[1] * ri, uy = q[2] * ri, uz = q[3] * ri;
     // cotangent of divB

@@ -28,6 +28,7 @@ struct AdjState {
   double *q[6] = {};          // stage states Q0..Q5 of one step (tile layout)
   double *b[6] = {};          // cotangents
   double *g = nullptr;        // Dbar / wbar per point
+  double *divb = nullptr;     // divB per point of the evaluation being reversed
   double *part = nullptr;     // per tile partial sums
   double *gm = nullptr;       // gmax before each of the 5 stages (device)
   unsigned long long *slots = nullptr;   // scratch maxima of recomputed evaluations
@@ -49,7 +50,7 @@ static void adj_free(AdjState *A) {
   if (!A) return;
   for (double *p : A->q) if (p) (void)hipFree(p);
   for (double *p : A->b) if (p) (void)hipFree(p);
-  for (void *p : {(void *)A->g, (void *)A->part, (void *)A->gm, (void *)A->slots,
+  for (void *p : {(void *)A->g, (void *)A->divb, (void *)A->part, (void *)A->gm, (void *)A->slots,
                   (void *)A->dcell, (void *)A->wq, (void *)A->amax, (void *)A->aloc})
     if (p) (void)hipFree(p);
   for (double *p : A->ck) if (p) (void)hipFree(p);
@@ -122,8 +123,7 @@ static int adj_alloc(sd2d_ctx *c, AdjState *A, int nck) {
   if (!A->aloc) HIPCHK(c, hipMalloc(&A->aloc, 2 * sizeof(int)));
   if (!A->slots) HIPCHK(c, hipMalloc(&A->slots, 2 * NSLOT * sizeof(unsigned long long)));
   if (!A->dcell) HIPCHK(c, hipMalloc(&A->dcell, (size_t)c->ntile * T * sizeof(double)));
-  if (!c->nablaf)
-    HIPCHK(c, hipMalloc(&c->nablaf, 2 * (size_t)c->ncell * 9 * N2 * sizeof(double)));
+  if (!A->divb) HIPCHK(c, hipMalloc(&A->divb, (size_t)c->ntile * N2 * T * sizeof(double)));
   while ((int)A->ck.size() < nck) {
     double *p = nullptr;
     HIPCHK(c, hipMalloc(&p, n * sizeof(double)));
@@ -203,7 +203,7 @@ static int adj_recompute_step(sd2d_ctx *c, const std::array<double, 4> &m) {
 static AdjArgs adj_args(sd2d_ctx *c, const double *q, double *qbar) {
   AdjState *A = c->adj;
   AdjArgs a{};
-  a.q = q; a.rscale = 1.0; a.nabla_f = c->nablaf; a.met = c->met; a.ftab = c->ftab;
+  a.q = q; a.rscale = 1.0; a.divb = A->divb; a.met = c->met; a.ftab = c->ftab;
   a.s2c = c->s2c; a.phm = c->phm; a.g = A->g; a.qbar = qbar; a.part = A->part;
   a.alpha = c->ph.alpha; a.ncell = c->ncell; a.ntile = c->ntile;
   return a;
@@ -242,16 +242,16 @@ static int adj_argmax_push(sd2d_ctx *c, const double *q, double *qbar, int kind,
 static int adj_resid_vjp(sd2d_ctx *c, const double *q, const double *gm, const double *rbar,
                          double rscale, double *qbar, double *acc) {
   AdjState *A = c->adj;
-  // divB of q: the compresid form of the stage kernel, with scratch maxima
+  // divB of q, as the stepping kernel forms it for the Powell source, with
+  // scratch maxima
   StageArgs s = base_args(c);
   s.qin = q;
   s.gmax = gm;
   s.slots = A->slots;
-  s.resid_f = c->stagef;
-  s.nabla_f = c->nablaf;
-  HIPCHK(c, launch_stage(c, s, M_COMPAT));
+  s.divb = A->divb;
+  HIPCHK(c, launch_stage(c, s, M_DIVB));
   AdjArgs a{};
-  a.q = q; a.rbar = rbar; a.rscale = rscale; a.nabla_f = c->nablaf; a.met = c->met;
+  a.q = q; a.rbar = rbar; a.rscale = rscale; a.divb = A->divb; a.met = c->met;
   a.ftab = c->ftab; a.s2c = c->s2c; a.phm = c->phm; a.gm = gm; a.g = A->g; a.qbar = qbar;
   a.part = A->part; a.alpha = c->ph.alpha; a.ncell = c->ncell; a.ntile = c->ntile;
   switch (c->N) {

@@ -56,6 +56,7 @@ struct StageArgs {
   double *nabla_f;          // nablaQs(9,2,N,N,NCELL) Fortran layout, or null
   const double *avtab;      // hlow[isl][is] then hhigh[is][isl] (AV.f90:18-63)
   const double *visres;     // viscous divergence dFv + dGv (tile layout), M_VIS
+  double *divb;             // divB per solution point [tile][p][ci], M_DIVB
   const int *s2c;           // slot -> mesh cell, for the cell-indexed outputs
   // per tile (member << 6 | valid cells), member 0 for one run; an
   // ensemble of runs also has per-member dt (qout then uses cr * dt, null
@@ -73,8 +74,11 @@ struct StageArgs {
 // that the stage update is straight-line code.
 // M_VIS adds the viscous divergence of the residual evaluation (formed by
 // the kernels of sd2d_visc.hpp) before the Jacobian scaling.
+// M_DIVB (no RK update, no Fortran-layout output) stores only divB at every
+// solution point, in tile layout: what the adjoint's reverse sweep of the
+// Powell source needs, with the arithmetic of the stepping kernel.
 enum { M_RK = 1, M_AV = 2, M_NORMS = 4, M_COMPAT = 8, M_AUX1 = 16, M_AUX4 = 32,
-       M_RSTORE = 64, M_VIS = 128 };
+       M_RSTORE = 64, M_VIS = 128, M_DIVB = 256 };
 
 // XCD-aware block order: consecutive positions of the tile launch order go
 // to the same XCD (and its L2), so the neighbour lines a tile reads were
@@ -348,6 +352,8 @@ __global__ __launch_bounds__(STAGE_THREADS, 2) void stage_kernel(StageArgs a, Op
   constexpr int T = G::CPW, N2 = G::N2, NF = N + 1;
   constexpr bool AV = (MODE & M_AV) != 0, NORMS = (MODE & M_NORMS) != 0;
   constexpr bool COMPAT = (MODE & M_COMPAT) != 0, RK = (MODE & M_RK) != 0;
+  constexpr bool DIVB = (MODE & M_DIVB) != 0;
+  static_assert(!DIVB || (!RK && !COMPAT), "M_DIVB is a mode of its own");
   constexpr int NAUX = (MODE & M_AUX4) ? 4 : ((MODE & M_AUX1) ? 1 : 0);
   constexpr bool RSTORE = (MODE & M_RSTORE) != 0, VIS = (MODE & M_VIS) != 0;
   static_assert(N * T <= 64 && STAGE_THREADS == 128, "one wavefront per line family");
@@ -681,6 +687,7 @@ __global__ __launch_bounds__(STAGE_THREADS, 2) void stage_kernel(StageArgs a, Op
 #pragma unroll
         for (int v = V0; v < V1; ++v) a.resid_f[v + 9 * pf] = R[v];
       }
+      if (DIVB && V0 == 0) a.divb[(size_t)tile * N2 * T + (size_t)p * T + ci] = divB;
       if (RK) {
         const size_t off = (size_t)tile * G::QT + ci;
         // several registers (FiveStageRk's last stage): their weighted sum,
