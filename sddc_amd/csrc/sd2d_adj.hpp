@@ -158,6 +158,175 @@ __device__ __forceinline__ void adj_trace(const double *qb, int p0, int ps, cons
   }
 }
 
+// ---- reverse sweeps of the pointwise fluxes (FP64) -------------------------------
+// With Fb the cotangent of a flux, these accumulate (dF/dq)^T Fb and
+// (dF/dc_h)^T Fb, the forward values being formed as iflux_n / rusanov
+// form them.  Derivatives of |x| take the sign of x (+1 at 0), of sqrt(x)
+// 1/(2 sqrt(x)), as the dual numbers of sd2d_dual.hpp do; the running
+// maxima outputs are not differentiated here (adj_argmax_kernel).
+
+// iflux_n (getrusanov.f90:127-193 contracted with (sx, sy))
+__device__ __forceinline__ void iflux_n_vjp(const double *q, double c_h, double sx, double sy,
+                                            const double *Fb, double *qb, double &chb) {
+  constexpr double g1 = GAM - 1.0;
+  const double rho = q[0], rhou = q[1], rhov = q[2], rhow = q[3], rhoE = q[4];
+  const double Bx = q[5], By = q[6], Bz = q[7], psi = q[8];
+  const double ri = rcp(rho);
+  const double u = rhou * ri, v = rhov * ri, w = rhow * ri;
+  const double B2 = Bx * Bx + By * By + Bz * Bz;
+  const double pr = g1 * (rhoE - 0.5 * (rhou * u + rhov * v + rhow * w) - 0.5 * B2 - 0.5 * psi * psi);
+  const double pt = pr + 0.5 * B2;
+  const double uB = u * Bx + v * By + w * Bz;
+  const double ef = rhoE - 0.5 * psi * psi + pt;
+  const double un = u * sx + v * sy;
+  const double bn = Bx * sx + By * sy;
+  const double cp = c_h * psi;
+  // the nine outputs (iflux_n), backwards
+  double rhob = 0.0, rhoub = Fb[0] * sx + Fb[1] * un, rhovb = Fb[0] * sy + Fb[2] * un;
+  double rhowb = Fb[3] * un, rhoEb = 0.0, psib = 0.0;
+  double Bxb = -Fb[1] * bn, Byb = -Fb[2] * bn, Bzb = -Fb[3] * bn + Fb[7] * un;
+  double ub = 0.0, vb = 0.0, wb = -Fb[7] * bn;
+  double unb = Fb[1] * rhou + Fb[2] * rhov + Fb[3] * rhow + Fb[4] * ef + Fb[7] * Bz;
+  double ptb = Fb[1] * sx + Fb[2] * sy;
+  double bnb = -Fb[1] * Bx - Fb[2] * By - Fb[3] * Bz + Fb[4] * (cp - uB) - Fb[7] * w + Fb[8] * c_h;
+  const double efb = Fb[4] * un;
+  const double uBb = -Fb[4] * bn;
+  const double cpb = Fb[4] * bn + Fb[5] * sx + Fb[6] * sy;
+  const double xb = Fb[5] * sy - Fb[6] * sx;        // x = v*Bx - u*By
+  double ch = Fb[8] * bn;
+  // intermediates, backwards
+  vb += xb * Bx; Bxb += xb * v; ub -= xb * By; Byb -= xb * u;
+  ch += cpb * psi; psib += cpb * c_h;
+  Bxb += bnb * sx; Byb += bnb * sy;
+  ub += unb * sx; vb += unb * sy;
+  rhoEb += efb; psib -= efb * psi; ptb += efb;
+  ub += uBb * Bx; Bxb += uBb * u; vb += uBb * By; Byb += uBb * v; wb += uBb * Bz; Bzb += uBb * w;
+  const double prb = ptb;
+  double B2b = 0.5 * ptb;
+  rhoEb += g1 * prb;
+  rhoub -= 0.5 * g1 * prb * u; ub -= 0.5 * g1 * prb * rhou;
+  rhovb -= 0.5 * g1 * prb * v; vb -= 0.5 * g1 * prb * rhov;
+  rhowb -= 0.5 * g1 * prb * w; wb -= 0.5 * g1 * prb * rhow;
+  B2b -= 0.5 * g1 * prb; psib -= g1 * prb * psi;
+  Bxb += 2.0 * Bx * B2b; Byb += 2.0 * By * B2b; Bzb += 2.0 * Bz * B2b;
+  rhoub += ub * ri; rhovb += vb * ri; rhowb += wb * ri;
+  rhob -= (ub * rhou + vb * rhov + wb * rhow) * ri * ri;
+  qb[0] = rhob; qb[1] = rhoub; qb[2] = rhovb; qb[3] = rhowb; qb[4] = rhoEb;
+  qb[5] = Bxb; qb[6] = Byb; qb[7] = Bzb; qb[8] = psib;
+  chb += ch;
+}
+
+// one side's physical flux of rusanov (Fl or Fr) backwards, with Fb its
+// cotangent and pgb / Vnb the cotangents its gas pressure and normal
+// velocity receive from the wave speed; accumulates into Qb and chb
+__device__ __forceinline__ void rusanov_side_vjp(const double *Q, double ux, double uy, double c_h,
+                                                 const double *Fb, double pgb, double Vnb,
+                                                 double *Qb, double &chb) {
+  constexpr double g1 = GAM - 1.0;
+  const double ri = rcp(Q[0]);
+  const double u = ri * Q[1], v = ri * Q[2], w = ri * Q[3];
+  const double B2 = Q[5] * Q[5] + Q[6] * Q[6] + Q[7] * Q[7];
+  const double pg = g1 * (Q[4] - 0.5 * Q[0] * (u * u + v * v + w * w) - B2 * 0.5 - 0.5 * Q[8] * Q[8]);
+  const double pt = pg + 0.5 * B2;
+  const double Vn = u * ux + v * uy;
+  const double Bn = Q[5] * ux + Q[6] * uy;
+  const double e = Q[4] - 0.5 * Q[8] * Q[8] + pt;
+  const double uB = u * Q[5] + v * Q[6] + w * Q[7];
+  Vnb += Fb[0] * Q[0] + Fb[1] * Q[1] + Fb[2] * Q[2] + Fb[3] * Q[3] + Fb[4] * e + Fb[7] * Q[7];
+  Qb[0] += Fb[0] * Vn; Qb[1] += Fb[1] * Vn; Qb[2] += Fb[2] * Vn; Qb[3] += Fb[3] * Vn;
+  Qb[7] += Fb[7] * Vn;
+  double ptb = Fb[1] * ux + Fb[2] * uy;
+  const double eb = Fb[4] * Vn;
+  Qb[4] += eb; Qb[8] -= eb * Q[8]; ptb += eb;
+  const double Bnb = -Fb[1] * Q[5] - Fb[2] * Q[6] - Fb[3] * Q[7] - Fb[4] * uB + Fb[4] * c_h * Q[8] -
+                     Fb[7] * w + Fb[8] * c_h;
+  Qb[5] -= Fb[1] * Bn; Qb[6] -= Fb[2] * Bn; Qb[7] -= Fb[3] * Bn;
+  const double uBb = -Fb[4] * Bn;
+  chb += Fb[4] * Q[8] * Bn + Fb[5] * Q[8] * ux + Fb[6] * Q[8] * uy + Fb[8] * Bn;
+  Qb[8] += Fb[4] * c_h * Bn + Fb[5] * c_h * ux + Fb[6] * c_h * uy;
+  const double xb = Fb[5] * uy - Fb[6] * ux;        // x = v*Q5 - u*Q6
+  double ub = 0.0, vb = 0.0, wb = -Fb[7] * Bn;
+  vb += xb * Q[5]; Qb[5] += xb * v; ub -= xb * Q[6]; Qb[6] -= xb * u;
+  ub += uBb * Q[5]; Qb[5] += uBb * u; vb += uBb * Q[6]; Qb[6] += uBb * v;
+  wb += uBb * Q[7]; Qb[7] += uBb * w;
+  Qb[5] += Bnb * ux; Qb[6] += Bnb * uy;
+  ub += Vnb * ux; vb += Vnb * uy;
+  pgb += ptb;
+  double B2b = 0.5 * ptb;
+  Qb[4] += g1 * pgb;
+  Qb[0] -= 0.5 * g1 * pgb * (u * u + v * v + w * w);
+  ub -= g1 * pgb * Q[0] * u; vb -= g1 * pgb * Q[0] * v; wb -= g1 * pgb * Q[0] * w;
+  B2b -= 0.5 * g1 * pgb; Qb[8] -= g1 * pgb * Q[8];
+  Qb[5] += 2.0 * Q[5] * B2b; Qb[6] += 2.0 * Q[6] * B2b; Qb[7] += 2.0 * Q[7] * B2b;
+  Qb[1] += ub * ri; Qb[2] += vb * ri; Qb[3] += wb * ri;
+  Qb[0] -= (ub * Q[1] + vb * Q[2] + wb * Q[3]) * ri * ri;
+}
+
+// rusanov (getrusanov.f90:1-125): Fn = sc (Fl + Fr - ev (QR - QL)) with the
+// fast magneto-acoustic speed ev of the averaged state; QLb, QRb are set
+__device__ __forceinline__ void rusanov_vjp(const double *QL, const double *QR, double nx,
+                                            double ny, int sign_l, double c_h, const double *Fnb,
+                                            double *QLb, double *QRb, double &chb) {
+  constexpr double g1 = GAM - 1.0;
+  const double mag = sqrt(nx * nx + ny * ny);
+  const double im = double(sign_l) * rcp(mag);
+  const double ux = nx * im, uy = ny * im;
+  const double ril = rcp(QL[0]), rir = rcp(QR[0]);
+  const double ul = ril * QL[1], vl = ril * QL[2], wl = ril * QL[3];
+  const double ur = rir * QR[1], vr = rir * QR[2], wr = rir * QR[3];
+  const double B2l = QL[5] * QL[5] + QL[6] * QL[6] + QL[7] * QL[7];
+  const double B2r = QR[5] * QR[5] + QR[6] * QR[6] + QR[7] * QR[7];
+  const double pgl = g1 * (QL[4] - 0.5 * QL[0] * (ul * ul + vl * vl + wl * wl) - B2l * 0.5 -
+                           0.5 * QL[8] * QL[8]);
+  const double pgr = g1 * (QR[4] - 0.5 * QR[0] * (ur * ur + vr * vr + wr * wr) - B2r * 0.5 -
+                           0.5 * QR[8] * QR[8]);
+  const double Vnl = ul * ux + vl * uy, Vnr = ur * ux + vr * uy;
+  const double bx = 0.5 * (QL[5] + QR[5]), by = 0.5 * (QL[6] + QR[6]), bz = 0.5 * (QL[7] + QR[7]);
+  const double irs = rcp(QL[0] + QR[0]);
+  const double ca = GAM * (pgl + pgr) * irs;
+  const double bb = bx * bx + by * by + bz * bz;
+  const double cd = 2.0 * bb * irs;
+  const double bn = bx * ux + by * uy;
+  const double cn = 2.0 * bn * bn * irs;
+  const double cad = ca + cd;
+  const double ct = sqrt(cad * cad - 4.0 * ca * cn);
+  const double cf = sqrt(0.5 * (cad + ct));
+  const double sV = Vnl + Vnr;
+  const double ev = 0.5 * fabs(sV) + cf;
+  const double sc = 0.5 * mag * double(sign_l);
+  double G[9], evb = 0.0;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    G[k] = sc * Fnb[k];
+    evb -= G[k] * (QR[k] - QL[k]);
+    QLb[k] = G[k] * ev;
+    QRb[k] = -G[k] * ev;
+  }
+  // ev = |Vnl + Vnr|/2 + cf, cf = sqrt((cad + ct)/2), ct = sqrt(cad^2 - 4 ca cn)
+  const double sb = 0.5 * (sV < 0.0 ? -evb : evb);
+  const double yb = evb / (2.0 * cf);
+  double cadb = 0.5 * yb;
+  const double ctb = 0.5 * yb;
+  const double zb = ctb / (2.0 * ct);
+  cadb += 2.0 * cad * zb;
+  const double cab = -4.0 * cn * zb + cadb;
+  const double cnb = -4.0 * ca * zb;
+  const double cdb = cadb;
+  const double bnb = 4.0 * bn * irs * cnb;
+  double irsb = 2.0 * bn * bn * cnb + 2.0 * bb * cdb + GAM * (pgl + pgr) * cab;
+  const double bxb = bnb * ux + 4.0 * bx * irs * cdb;
+  const double byb = bnb * uy + 4.0 * by * irs * cdb;
+  const double bzb = 4.0 * bz * irs * cdb;
+  const double pgb = GAM * irs * cab;
+  const double d0 = -irs * irs * irsb;
+  QLb[0] += d0; QRb[0] += d0;
+  QLb[5] += 0.5 * bxb; QRb[5] += 0.5 * bxb;
+  QLb[6] += 0.5 * byb; QRb[6] += 0.5 * byb;
+  QLb[7] += 0.5 * bzb; QRb[7] += 0.5 * bzb;
+  rusanov_side_vjp(QL, ux, uy, c_h, G, pgb, sb, QLb, chb);
+  rusanov_side_vjp(QR, ux, uy, c_h, G, pgb, sb, QRb, chb);
+}
+
 // The trace cotangent of flux point r of a line (own side only), and the
 // c_h / free-stream partial sums of the points this cell owns.
 template <int N>
@@ -177,17 +346,7 @@ __device__ void adj_point_flux(const AdjArgs &a, const Ops &ops, double c_h, int
   adj_flux_cot<N>(a.g + (size_t)tile * 10 * N2 * T + ci, p0, ps, r, ops, Fb, Wb);
   if (r != 0 && r != N) {
     // interior flux point (compflux.f90:53-104) and the divB weight of the trace
-#pragma unroll 1
-    for (int k = 0; k < 10; ++k) {
-      Dual qd[9], Fd[9], umd(0.0);
-#pragma unroll
-      for (int v = 0; v < 9; ++v) qd[v] = Dual(t[v], v == k ? 1.0 : 0.0);
-      iflux_n<Dual>(qd, Dual(c_h, k == 9 ? 1.0 : 0.0), Dual(sx), Dual(sy), Fd, umd);
-      double acc = 0.0;
-#pragma unroll
-      for (int v = 0; v < 9; ++v) acc += Fb[v] * Fd[v].d;
-      if (k < 9) Qt[k] = acc; else chb += acc;
-    }
+    iflux_n_vjp(t, c_h, sx, sy, Fb, Qt, chb);
     Qt[5] += Wb * sx;
     Qt[6] += Wb * sy;
     return;
@@ -223,22 +382,14 @@ __device__ void adj_point_flux(const AdjArgs &a, const Ops &ops, double c_h, int
     double Gb[9];
 #pragma unroll
     for (int v = 0; v < 9; ++v) Gb[v] = fo * Fb[v] + fn * Fnb[v];
-    const int nk = left ? 10 : 9;     // the left cell also takes the c_h part
-#pragma unroll 1
-    for (int kk = 0; kk < nk; ++kk) {
-      Dual QL[9], QR[9], Fl[9], eig(0.0), um(0.0);
+    {
+      // the face flux backwards; this cell keeps its own side's cotangent,
+      // and the left cell also takes the c_h part
+      double QLb[9], QRb[9], chf = 0.0;
+      rusanov_vjp(left ? t : qn, left ? qn : t, nsg * sx, nsg * sy, sl, c_h, Gb, QLb, QRb, chf);
 #pragma unroll
-      for (int v = 0; v < 9; ++v) {
-        const Dual own(t[v], v == kk ? 1.0 : 0.0), nb(qn[v]);
-        QL[v] = left ? own : nb;
-        QR[v] = left ? nb : own;
-      }
-      rusanov<Dual>(QL, QR, Dual(nsg * sx), Dual(nsg * sy), sl, Dual(c_h, kk == 9 ? 1.0 : 0.0),
-                    Fl, eig, um);
-      double acc = 0.0;
-#pragma unroll
-      for (int v = 0; v < 9; ++v) acc += Gb[v] * Fl[v].d;
-      if (kk < 9) Qt[kk] = acc; else chb += acc;
+      for (int v = 0; v < 9; ++v) Qt[v] = left ? QLb[v] : QRb[v];
+      if (left) chb += chf;
     }
     // divB weights of the averaged state Qv = (QL + QR)/2 seen by both cells
     const double wq = 0.5 * (Wb + sig * Wnb);
