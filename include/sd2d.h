@@ -395,12 +395,28 @@ int sd2d_ens_nonfinite(sd2d_ens *ens, int32_t *flags);
  *               current c_h (sd2d_compresid's contract): qbar = (dR/dQ)^T
  *               rbar, *chbar = (dR/dc_h)^T rbar and pbar[6] (Fortran
  *               layouts; outputs may be NULL).
+ * sd2d_adj_save / sd2d_adj_load  the checkpoints of the last sd2d_fwd as
+ *               files, so that the backward pass can run in another
+ *               context or process (SURVEY's restartable adjoint runs; no
+ *               counterpart in the reference, whose restart files
+ *               write_data.f90:44-74 they follow): <prefix>.NNNN.dat for
+ *               checkpoint NNNN = 0..ceil(nsteps/ckpt), each a reference
+ *               restart file with the maxima sidecar of
+ *               sd2d_write_checkpoint, and <prefix>.adj (8 bytes "SD2DADJ1",
+ *               int32 N, NCELL, nsteps, ckpt, final iter, float64 J, final
+ *               ctime).  sd2d_adj_load leaves the context as sd2d_fwd does
+ *               (the final state, maxima, ctime and iter), ready for
+ *               sd2d_vjp, whose results are then bitwise those of the
+ *               context that ran the forward; a mesh or order other than
+ *               the files' is SD2D_EINVAL.
  */
 int sd2d_fwd(sd2d_ctx *ctx, int32_t nsteps, int32_t ckpt, double *objective);
 int sd2d_vjp(sd2d_ctx *ctx, const double *qbar_final, double jbar, double *qbar_init,
              double *pbar, double *chbar);
 int sd2d_compresid_vjp(sd2d_ctx *ctx, const double *rbar, double *qbar, double *chbar,
                        double *pbar);
+int sd2d_adj_save(sd2d_ctx *ctx, const char *prefix);
+int sd2d_adj_load(sd2d_ctx *ctx, const char *prefix);
 
 #ifdef __cplusplus
 }

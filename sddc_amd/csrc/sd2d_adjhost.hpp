@@ -36,6 +36,8 @@ struct AdjState {
   double *wq = nullptr;       // face quadrature weights
   std::vector<double *> ck;   // checkpoint states (index = step / ckpt) + final state
   std::vector<std::array<double, 4>> ckm;   // their maxima
+  std::vector<int32_t> cki;   // their iter and ctime
+  std::vector<double> ckt;
   std::vector<double *> seg;  // step inputs of one segment
   std::vector<std::array<double, 4>> segm;
   double fin_ctime = 0.0;
@@ -326,6 +328,15 @@ static int adj_step_vjp(sd2d_ctx *c, double *pacc, double *chbar5) {
 
 extern "C" {
 
+// the wall-drag quadrature weights on the device, once per context
+static int adj_weights(sd2d_ctx *c, AdjState *A) {
+  if (A->wq) return SD2D_OK;
+  const std::vector<double> w = face_weights(c->xs.data(), c->N);
+  HIPCHK(c, hipMalloc(&A->wq, w.size() * sizeof(double)));
+  HIPCHK(c, hipMemcpy(A->wq, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice));
+  return SD2D_OK;
+}
+
 int sd2d_fwd(sd2d_ctx *c, int32_t nsteps, int32_t ckpt, double *objective) {
   if (!c) return fail(nullptr, SD2D_EINVAL, "ctx is NULL");
   if (nsteps <= 0 || ckpt <= 0) return fail(c, SD2D_EINVAL, "nsteps and ckpt must be positive");
@@ -343,17 +354,17 @@ int sd2d_fwd(sd2d_ctx *c, int32_t nsteps, int32_t ckpt, double *objective) {
   const int nck = (nsteps + ckpt - 1) / ckpt + 1;   // segment starts + the final state
   int r = adj_alloc(c, A, nck);
   if (r) return r;
-  if (!A->wq) {
-    const std::vector<double> w = face_weights(c->xs.data(), c->N);
-    HIPCHK(c, hipMalloc(&A->wq, w.size() * sizeof(double)));
-    HIPCHK(c, hipMemcpy(A->wq, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice));
-  }
+  if ((r = adj_weights(c, A))) return r;
+  A->cki.assign(nck, 0);
+  A->ckt.assign(nck, 0.0);
   const size_t nb = state_len(c) * sizeof(double);
   double J = 0.0;
   for (int n = 0; n < nsteps; ++n) {
     if (n % ckpt == 0) {
       HIPCHK(c, hipMemcpyAsync(A->ck[n / ckpt], c->q[c->cur], nb, hipMemcpyDeviceToDevice, c->stream));
       if ((r = adj_maxima_get(c, A->ckm[n / ckpt]))) return r;
+      A->cki[n / ckpt] = c->iter;
+      A->ckt[n / ckpt] = c->ctime;
     }
     c->iter += 1;
     if ((r = step_rk(c, false))) return r;
@@ -366,6 +377,8 @@ int sd2d_fwd(sd2d_ctx *c, int32_t nsteps, int32_t ckpt, double *objective) {
   if ((r = adj_maxima_get(c, A->ckm[nck - 1]))) return r;
   A->fin_ctime = c->ctime;
   A->fin_iter = c->iter;
+  A->cki[nck - 1] = c->iter;
+  A->ckt[nck - 1] = c->ctime;
   A->J = J / nsteps;
   A->have_fwd = true;
   if (objective) *objective = A->J;
@@ -427,6 +440,109 @@ int sd2d_vjp(sd2d_ctx *c, const double *qbar_final, double jbar, double *qbar_in
     return fail(c, SD2D_EINVAL, "adjoint: the flux point of a running maximum was not found");
   // the context returns to the final state of sd2d_fwd
   HIPCHK(c, hipMemcpyAsync(c->q[c->cur], A->ck[nck - 1], nb, hipMemcpyDeviceToDevice, c->stream));
+  if ((r = adj_maxima_set(c, A->ckm[nck - 1]))) return r;
+  c->ctime = A->fin_ctime;
+  c->iter = A->fin_iter;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return SD2D_OK;
+}
+
+// The adjoint checkpoints of the last sd2d_fwd as files: <prefix>.NNNN.dat
+// (NNNN = 0..nck-1) in the reference's restart format with the maxima
+// sidecar of sd2d_write_checkpoint, and <prefix>.adj: 8 bytes "SD2DADJ1",
+// int32 N, NCELL, nsteps, ckpt, final iter, float64 J, final ctime.
+static std::string adj_ck_path(const char *prefix, int k) {
+  char tail[16];
+  std::snprintf(tail, sizeof(tail), ".%04d.dat", k);
+  return std::string(prefix) + tail;
+}
+
+int sd2d_adj_save(sd2d_ctx *c, const char *prefix) {
+  if (!c || !prefix) return fail(c, SD2D_EINVAL, "NULL argument");
+  AdjState *A = c->adj;
+  if (!A || !A->have_fwd) return fail(c, SD2D_EINVAL, "sd2d_adj_save needs a preceding sd2d_fwd");
+  const int nck = (A->nsteps + A->ckpt - 1) / A->ckpt + 1;
+  const size_t n = (size_t)c->ncell * 9 * c->N * c->N;
+  std::vector<double> q(n);
+  for (int k = 0; k < nck; ++k) {
+    int r = t2f(c, A->ck[k], c->stagef);
+    if (r) return r;
+    HIPCHK(c, hipMemcpyAsync(q.data(), c->stagef, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const std::string path = adj_ck_path(prefix, k);
+    const sd2d_maxima m{A->ckm[k][0], A->ckm[k][1], A->ckm[k][2], A->ckm[k][3]};
+    if (!write_restart_file(path.c_str(), A->cki[k], A->ckt[k], c->N, c->ncell, q.data(), nullptr) ||
+        !write_max_file(path.c_str(), A->cki[k], A->ckt[k], m))
+      return fail(c, SD2D_EINVAL, "write failed: " + path);
+  }
+  const std::string mp = std::string(prefix) + ".adj";
+  FILE *fh = std::fopen(mp.c_str(), "wb");
+  if (!fh) return fail(c, SD2D_EINVAL, "cannot write " + mp);
+  const int32_t hv[5] = {c->N, c->ncell, A->nsteps, A->ckpt, A->fin_iter};
+  const double dv[2] = {A->J, A->fin_ctime};
+  const bool ok = std::fwrite("SD2DADJ1", 1, 8, fh) == 8 && std::fwrite(hv, sizeof(int32_t), 5, fh) == 5 &&
+                  std::fwrite(dv, sizeof(double), 2, fh) == 2;
+  if (std::fclose(fh) != 0 || !ok) return fail(c, SD2D_EINVAL, "write failed: " + mp);
+  return SD2D_OK;
+}
+
+int sd2d_adj_load(sd2d_ctx *c, const char *prefix) {
+  if (!c || !prefix) return fail(c, SD2D_EINVAL, "NULL argument");
+  SD2D_NOT_MID_STEP(c, "sd2d_adj_load");
+  if (c->p.precision != SD2D_FP64 || c->p.k_stage != 3 || c->vis || c->nmem > 1)
+    return fail(c, SD2D_EUNSUPPORTED,
+                "the adjoint covers the FP64 inviscid path with RK_S5P3 (k_stage 3)");
+  const std::string mp = std::string(prefix) + ".adj";
+  FILE *fh = std::fopen(mp.c_str(), "rb");
+  if (!fh) return fail(c, SD2D_EINVAL, "cannot open " + mp);
+  char magic[8];
+  int32_t hv[5];
+  double dv[2];
+  const bool ok = std::fread(magic, 1, 8, fh) == 8 && std::fread(hv, sizeof(int32_t), 5, fh) == 5 &&
+                  std::fread(dv, sizeof(double), 2, fh) == 2 && std::fgetc(fh) == EOF;
+  std::fclose(fh);
+  if (!ok || std::memcmp(magic, "SD2DADJ1", 8) != 0)
+    return fail(c, SD2D_EINVAL, "not an adjoint checkpoint manifest: " + mp);
+  if (hv[0] != c->N || hv[1] != c->ncell || hv[2] <= 0 || hv[3] <= 0)
+    return fail(c, SD2D_EINVAL, "the adjoint checkpoints of " + mp + " belong to another mesh or order");
+  if (!c->adj) c->adj = new (std::nothrow) AdjState();
+  AdjState *A = c->adj;
+  if (!A) return fail(c, SD2D_ENOMEM, "host allocation failed");
+  A->have_fwd = false;
+  A->nsteps = hv[2];
+  A->ckpt = hv[3];
+  const int nck = (A->nsteps + A->ckpt - 1) / A->ckpt + 1;
+  int r = adj_alloc(c, A, nck);
+  if (r) return r;
+  if ((r = adj_weights(c, A))) return r;
+  A->cki.assign(nck, 0);
+  A->ckt.assign(nck, 0.0);
+  const size_t n = (size_t)c->ncell * 9 * c->N * c->N;
+  for (int k = 0; k < nck; ++k) {
+    const std::string path = adj_ck_path(prefix, k);
+    int32_t it = 0;
+    double t = 0.0;
+    std::vector<double> q;
+    sd2d_maxima m{};
+    std::string e = read_restart_file(path.c_str(), c->N, c->ncell, it, t, q);
+    if (e.empty()) e = read_max_file(path.c_str(), it, t, m);
+    if (!e.empty()) return fail(c, SD2D_EINVAL, e);
+    HIPCHK(c, hipMemcpyAsync(c->stagef, q.data(), n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    if ((r = f2t(c, c->stagef, A->ck[k]))) return r;
+    HIPCHK(c, hipStreamSynchronize(c->stream));   // q is reused by the next file
+    A->ckm[k] = {m.eigvmax, m.umax, m.eigvmax_g, m.umax_g};
+    A->cki[k] = it;
+    A->ckt[k] = t;
+  }
+  A->J = dv[0];
+  A->fin_ctime = dv[1];
+  A->fin_iter = hv[4];
+  if (A->cki[nck - 1] != A->fin_iter || std::memcmp(&A->ckt[nck - 1], &A->fin_ctime, sizeof(double)) != 0)
+    return fail(c, SD2D_EINVAL, "the last adjoint checkpoint is not the final state of " + mp);
+  A->have_fwd = true;
+  // the context holds the final state of the forward run, as after sd2d_fwd
+  HIPCHK(c, hipMemcpyAsync(c->q[c->cur], A->ck[nck - 1], state_len(c) * sizeof(double),
+                           hipMemcpyDeviceToDevice, c->stream));
   if ((r = adj_maxima_set(c, A->ckm[nck - 1]))) return r;
   c->ctime = A->fin_ctime;
   c->iter = A->fin_iter;

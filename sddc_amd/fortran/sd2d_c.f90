@@ -66,7 +66,7 @@ module sd2d_c
        sd2d_ens_set_time, sd2d_ens_get_time, sd2d_ens_write_checkpoint, &
        sd2d_ens_read_checkpoint, sd2d_ens_step, sd2d_ens_tune, &
        sd2d_ens_set_check_finite, sd2d_ens_nonfinite, sd2d_fwd, sd2d_vjp, &
-       sd2d_compresid_vjp, sd2d_error_message
+       sd2d_compresid_vjp, sd2d_adj_save, sd2d_adj_load, sd2d_error_message
 
   interface
      integer(c_int) function sd2d_create(mesh, params, ctx) bind(C, name="sd2d_create")
@@ -339,6 +339,19 @@ module sd2d_c
        real(c_double), value :: jbar
        real(c_double), intent(out), optional :: qbar_init(*), pbar(*), chbar(*)
      end function sd2d_vjp
+
+     ! the adjoint checkpoints of the last sd2d_fwd as files, and back
+     integer(c_int) function sd2d_adj_save(ctx, prefix) bind(C, name="sd2d_adj_save")
+       import :: c_int, c_ptr, c_char
+       type(c_ptr), value :: ctx
+       character(kind=c_char), intent(in) :: prefix(*)
+     end function sd2d_adj_save
+
+     integer(c_int) function sd2d_adj_load(ctx, prefix) bind(C, name="sd2d_adj_load")
+       import :: c_int, c_ptr, c_char
+       type(c_ptr), value :: ctx
+       character(kind=c_char), intent(in) :: prefix(*)
+     end function sd2d_adj_load
 
      integer(c_int) function sd2d_compresid_vjp(ctx, rbar, qbar, chbar, pbar) &
           bind(C, name="sd2d_compresid_vjp")

@@ -33,7 +33,7 @@ EXPORTS = ("sd2d_create", "sd2d_destroy", "sd2d_last_error", "sd2d_set_state",
            "sd2d_ens_set_state", "sd2d_ens_get_state", "sd2d_ens_set_time",
            "sd2d_ens_get_time", "sd2d_ens_step", "sd2d_ens_tune",
            "sd2d_set_check_finite", "sd2d_ens_set_check_finite", "sd2d_ens_nonfinite",
-           "sd2d_fwd", "sd2d_vjp", "sd2d_compresid_vjp")
+           "sd2d_fwd", "sd2d_vjp", "sd2d_compresid_vjp", "sd2d_adj_save", "sd2d_adj_load")
 
 
 class Sd2dError(RuntimeError):
@@ -97,6 +97,8 @@ def load_library(path: str = LIBPATH):
     L.sd2d_fwd.argtypes = [vp, C.c_int32, C.c_int32, C.POINTER(C.c_double)]
     L.sd2d_vjp.argtypes = [vp, vp, C.c_double, vp, vp, vp]
     L.sd2d_compresid_vjp.argtypes = [vp, vp, vp, C.POINTER(C.c_double), vp]
+    L.sd2d_adj_save.argtypes = [vp, C.c_char_p]
+    L.sd2d_adj_load.argtypes = [vp, C.c_char_p]
     for f in EXPORTS:
         if f not in ("sd2d_destroy", "sd2d_last_error", "sd2d_ens_destroy",
                      "sd2d_ens_last_error"):
@@ -254,6 +256,16 @@ class Sd2d:
         self._check(self.L.sd2d_vjp(self.h, _ptr(qf), float(jbar), _ptr(q0), _ptr(pb), _ptr(ch)),
                     "sd2d_vjp")
         return q0, pb, ch
+
+    def adj_save(self, prefix):
+        """The checkpoints of the last fwd() as <prefix>.NNNN.dat restart
+        files (with maxima sidecars) and the manifest <prefix>.adj."""
+        self._check(self.L.sd2d_adj_save(self.h, str(prefix).encode()), "sd2d_adj_save")
+
+    def adj_load(self, prefix):
+        """Load what adj_save wrote: the context is then as after fwd(),
+        ready for vjp()."""
+        self._check(self.L.sd2d_adj_load(self.h, str(prefix).encode()), "sd2d_adj_load")
 
     def compresid_vjp(self, rbar):
         rb = np.asfortranarray(rbar, dtype=np.float64)

@@ -93,6 +93,49 @@ def test_step_vjp_against_fd(name):
     g.close()
 
 
+@pytest.mark.parametrize("name", ["ogrid4x8_N4", "ogrid_curved5x12_N5"])
+def test_adjoint_checkpoints_restart(tmp_path, name):
+    """The forward's checkpoints saved to files (sd2d_adj_save) and loaded
+    into a fresh context (sd2d_adj_load) give the backward pass of the
+    context that ran the forward bit for bit; every checkpoint file is a
+    reference restart file (the first one holds the initial state), and a
+    context of another mesh refuses the files."""
+    print(f"[gpu-adj] checkpoint files {name}", flush=True)
+    from sddc_amd import mesh as M
+    from sddc_amd.solver import Sd2dError
+    case, Q0 = make_case(name)
+    C = np.asfortranarray(np.random.default_rng(11).standard_normal(Q0.shape))
+    nsteps, ckpt = 7, 3                      # checkpoints at steps 0, 3, 6 and the final state
+    g = _gpu(case)
+    g.set_state(Q0, [0, 0, 0, 0])
+    g.set_time(0.25, 40)
+    J = g.fwd(nsteps, ckpt)
+    pre = str(tmp_path / "adj")
+    g.adj_save(pre)
+    qa, pa, ca = g.vjp(C, 2.0, want_chbar=True)
+    Qa, ma = g.get_state()
+    da = g.diag()
+    g.close()
+    q0, t0, it0 = M.read_restart(pre + ".0000.dat", case.N)
+    assert np.array_equal(q0, Q0) and t0 == 0.25 and it0 == 40
+    for k in range(4):
+        assert (tmp_path / f"adj.{k:04d}.dat.max").exists()
+    h = _gpu(case)
+    h.adj_load(pre)
+    Qh, mh = h.get_state()
+    assert np.array_equal(Qh, Qa) and np.array_equal(mh, ma)
+    assert h.diag()["iter"] == da["iter"] == 47 and h.diag()["ctime"] == da["ctime"]
+    qb, pbb, cb = h.vjp(C, 2.0, want_chbar=True)
+    assert np.array_equal(qa, qb) and np.array_equal(pa, pbb) and np.array_equal(ca, cb)
+    h.close()
+    other, _ = make_case("channel5x4_N3")
+    o = _gpu(other)
+    with pytest.raises(Sd2dError, match="another mesh"):
+        o.adj_load(pre)
+    o.close()
+    assert np.isfinite(J)
+
+
 def test_parameter_gradients_against_fd():
     """dJ/dMach and dJ/dtheta of the time-averaged wall drag (boundary
     ghosts and initial state) against central differences with h = 1e-6;
