@@ -46,7 +46,7 @@ struct sd2d_ctx {
   double *met = nullptr;
   int *ftab = nullptr;
   int *c2s = nullptr, *s2c = nullptr;          // cell <-> slot maps of the tile shape
-  double *avtab = nullptr;
+  double avf[NMAX * NMAX] = {};               // AV smoothing filter (kernel arguments)
   double *gmax = nullptr;                       // eigvmax_g, umax_g
   double *lastch = nullptr;                     // c_h of the last evaluation (per member)
   unsigned long long *slots = nullptr;
@@ -457,7 +457,7 @@ static StageArgs base_args(sd2d_ctx *c) {
   a.ftab = c->ftab;
   a.gmax = c->gmax;
   a.slots = c->slots;
-  a.avtab = c->avtab;
+  std::memcpy(a.avf, c->avf, sizeof(a.avf));
   a.ncell = c->ncell;
   a.ntile = c->ntile;
   a.sm_ind = c->sm_ind;
@@ -515,7 +515,6 @@ static int ctx_upload(sd2d_ctx *c, const HostTables &tb, int nwall, int nmem,
   const sd2d_params *p = &c->p;
   const std::vector<int> &ft = tb.ft;
   const std::vector<double> &met = tb.met;
-  const std::vector<double> &av = tb.av;
 
 #define CK(call)                                                                   \
   do {                                                                             \
@@ -555,7 +554,6 @@ static int ctx_upload(sd2d_ctx *c, const HostTables &tb, int nwall, int nmem,
   CK(hipMalloc(&c->ftab, ft.size() * sizeof(int)));
   CK(hipMalloc(&c->c2s, tb.c2s.size() * sizeof(int)));
   CK(hipMalloc(&c->s2c, tb.s2c.size() * sizeof(int)));
-  CK(hipMalloc(&c->avtab, av.size() * sizeof(double)));
   CK(hipMalloc(&c->gmax, 2 * (size_t)nmem * sizeof(double)));
   CK(hipMalloc(&c->lastch, (size_t)nmem * sizeof(double)));
   CK(hipMalloc(&c->nfflag, (size_t)nmem * sizeof(int)));
@@ -583,7 +581,8 @@ static int ctx_upload(sd2d_ctx *c, const HostTables &tb, int nwall, int nmem,
                     c->stream));
   CK(hipMemcpyAsync(c->s2c, tb.s2c.data(), tb.s2c.size() * sizeof(int), hipMemcpyHostToDevice,
                     c->stream));
-  CK(hipMemcpyAsync(c->avtab, av.data(), av.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  if (tb.av.size() != (size_t)N2) return fail(c, SD2D_EINVAL, "AV filter of the wrong size");
+  std::copy(tb.av.begin(), tb.av.end(), c->avf);
   for (int i = 0; i < 3; ++i) CK(hipMemsetAsync(c->q[i], 0, qn * sizeof(double), c->stream));
   CK(hipMemsetAsync(c->sm_ind, 0, 2 * (size_t)nc * sizeof(double), c->stream));
   if (tinfo) {
@@ -675,7 +674,7 @@ void sd2d_destroy(sd2d_ctx *c) {
   adj_free(c->adj);
   for (double *b : c->q) if (b) (void)hipFree(b);
   for (double *b : c->fsr) if (b) (void)hipFree(b);
-  void *ptrs[] = {c->met, c->metf, c->ftab, c->avtab, c->gmax, c->lastch, c->nfflag, c->slots, c->sm_ind, c->normbuf,
+  void *ptrs[] = {c->met, c->metf, c->ftab, c->gmax, c->lastch, c->nfflag, c->slots, c->sm_ind, c->normbuf,
                   c->normout, c->stagef, c->nablaf, c->gq, c->dei, c->visres, c->metw,
                   c->c2s, c->s2c, c->tinfo, c->dtm, c->phm};
   for (void *b : ptrs) if (b) (void)hipFree(b);

@@ -36,6 +36,17 @@ template <int N> struct Geo {
   static constexpr int QT = 9 * N2 * CPW;   // doubles per state tile
 };
 
+// Values handed between lanes of one wavefront through LDS: the LDS unit
+// performs a wavefront's operations in order, so no wait is needed between
+// one lane's store and another lane's load; the fences keep the compiler
+// from moving LDS accesses across this point.  Every lane of the wavefront
+// must reach it (the CPU emulation of the kernels syncs the wave here).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Interpolation / differentiation operators (initsetup.f90:262-319),
 // passed by value in the kernarg.
 struct Ops {

@@ -25,7 +25,7 @@ struct HostTables {
   std::vector<int> ft;       // [tile][k][{neighbour base, info}][ci]
   std::vector<int> tvalid;   // valid cells of each tile (its first slots)
   std::vector<double> met;   // [tile][metric slot][ci]
-  std::vector<double> av;    // hlow[isl][is], then hhigh[is][isl]
+  std::vector<double> av;    // AV smoothing filter F[is][k] (N x N)
   std::vector<double> metw;  // viscous path: other-family metrics at line ends
 };
 
@@ -517,15 +517,25 @@ inline int build_host_tables(const sd2d_mesh *m, const sd2d_params *p, HostTable
         }
       }
   }
-  // AV projections (AV.f90:18-63): hlow[isl][is], hhigh[is][isl]
-  t.av.assign((size_t)2 * N * (N - 1), 0.0);
+  // AV smoothing filter per direction (AV.f90:18-63): the projection onto
+  // the N-1 low-order points, hlow[isl][is] = h_is(Xs_low(isl)), followed by
+  // the interpolation back, hhigh[is][isl] = hlow_isl(Xs(is)), as one map
+  // F[is][k] = sum_isl hhigh[is][isl] hlow[isl][k]
+  t.av.assign((size_t)N * N, 0.0);
   if (m->xs_low) {
+    std::vector<double> hlow((size_t)(N - 1) * N), hhigh((size_t)N * (N - 1));
     for (int il = 0; il < N - 1; ++il)
       for (int is = 0; is < N; ++is)
-        t.av[il * N + is] = hostdetail::lagrange(m->xs, N, is, m->xs_low[il]);
+        hlow[il * N + is] = hostdetail::lagrange(m->xs, N, is, m->xs_low[il]);
     for (int is = 0; is < N; ++is)
       for (int il = 0; il < N - 1; ++il)
-        t.av[(N - 1) * N + is * (N - 1) + il] = hostdetail::lagrange(m->xs_low, N - 1, il, m->xs[is]);
+        hhigh[is * (N - 1) + il] = hostdetail::lagrange(m->xs_low, N - 1, il, m->xs[is]);
+    for (int is = 0; is < N; ++is)
+      for (int k = 0; k < N; ++k) {
+        double f = 0.0;
+        for (int il = 0; il < N - 1; ++il) f += hhigh[is * (N - 1) + il] * hlow[il * N + k];
+        t.av[is * N + k] = f;
+      }
   }
   return SD2D_OK;
 }

@@ -54,7 +54,12 @@ struct StageArgs {
   double *normbuf;          // per-tile sums of resid^2 and divB^2, or null
   double *resid_f;          // resid(9,N,N,NCELL) Fortran layout, or null
   double *nabla_f;          // nablaQs(9,2,N,N,NCELL) Fortran layout, or null
-  const double *avtab;      // hlow[isl][is] then hhigh[is][isl] (AV.f90:18-63)
+  // the AV sensor's smoothing filter per direction, F[i][k] at avf[i*N + k]:
+  // the projection onto the N-1 low-order points followed by the
+  // interpolation back to the N solution points (AV.f90:18-63) is one linear
+  // map F = Hhigh Hlow, the same along xi and eta (host-built, in the
+  // kernel arguments so that its entries are scalar loads)
+  double avf[NMAX * NMAX];
   const double *visres;     // viscous divergence dFv + dGv (tile layout), M_VIS
   double *divb;             // divB per solution point [tile][p][ci], M_DIVB
   const int *s2c;           // slot -> mesh cell, for the cell-indexed outputs
@@ -98,10 +103,12 @@ __device__ __forceinline__ int tile_of_block(int b, int nt, int rev) {
 // derivatives of the fluxes of up to XV variables [v][p][ci] handed from one
 // family to the other (eta -> xi: variables 0-4, then xi -> eta: 5-8), of
 // the divB weight [p][ci] and (AV) of the Jz weight [p][ci], then
-// reduction slots.  The AV sensor reuses the state area once the residual
-// is formed.
+// reduction slots.  The AV sensor of the eta-line wave reads rho from the
+// state area, keeps Jz^2 in the Jz area and its line sums in the fifth
+// variable of the exchange area, which the other wave no longer touches
+// once the second exchange round has begun.
 template <int N, int MODE> struct StageLds {
-  static constexpr int T = Geo<N>::CPW, N2 = N * N, NL = N - 1;
+  static constexpr int T = Geo<N>::CPW, N2 = N * N;
   static constexpr bool AV = (MODE & M_AV) != 0;
   static constexpr int XV = 5;                 // variables per exchange round
   static constexpr int Q0 = 0;
@@ -110,13 +117,8 @@ template <int N, int MODE> struct StageLds {
   static constexpr int XJ = XB + N2 * T;
   static constexpr int RED = XJ + (AV ? N2 * T : 0);
   static constexpr int TOTAL = RED + 4 * (STAGE_THREADS / 64);
-  // AV sensor scratch inside [Q0, XD): rho, Jz^2 [p][ci], low-order
-  // projections [2][NL*NL][ci], point errors [2][N2][ci]
-  static constexpr int A_RHO = Q0;
-  static constexpr int A_JZ = Q0 + N2 * T;
-  static constexpr int A_LOW = Q0 + 2 * N2 * T;
-  static constexpr int A_SE = A_LOW + 2 * NL * NL * T;
-  static_assert(A_SE + 2 * N2 * T <= XD, "AV scratch exceeds the state area");
+  static constexpr int A_ROW = XD + 4 * N2 * T;   // AV line sums [2][line][ci]
+  static_assert(XV == 5 && 2 * N <= N2, "AV line sums fit the fifth exchange variable");
 };
 
 template <int N, int MODE>
@@ -147,6 +149,72 @@ __device__ __forceinline__ double wave_sum(double x) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
   return x;
+}
+
+// AV smoothness sensor (AV.f90:14-83) of the cells of one tile, by one
+// wave of line threads.  The reference projects rho and Jz^2 of a cell onto
+// the (N-1)^2 low-order points and interpolates back; per direction that is
+// the filter F = Hhigh Hlow, so the smoothed values are (F x F) applied to
+// the cell's N x N values.  A thread filters the whole cell across its line
+// onto the line (ROW: the xi-line of row js = line; otherwise the eta-line
+// of column is = line), then along the line, and sums its points' terms
+// (smoothed - x)^2 / x^2; the N line sums of a cell meet in LDS and lane
+// line 0 of the cell takes log10 of each total.  rho and jz2 hold the
+// cell's values [p][ci] (p = is + N js) in LDS, rs and js2 the thread's own
+// points in line order; only lanes of this wave exchange values, so the
+// caller needs no workgroup barrier but must call this from every lane of
+// the wave (a wave-level LDS sync inside).  The sums run in another order
+// than the reference's, so values differ from it in the last bits of the
+// smoothed field.
+template <int N, int T, bool ROW>
+__device__ __forceinline__ void av_sensor_lines(const StageArgs &a, const double *rho,
+                                                const double *jz2, double *lineb,
+                                                const double (&rs)[N], const double (&js2)[N],
+                                                int line, int ci, int cell, bool valid) {
+  wave_lds_sync();   // the cell's Jz^2 (and rho) of the other lines are in LDS
+  if (valid) {
+    double y1[N], y2[N];
+#pragma unroll
+    for (int m = 0; m < N; ++m) {
+      double u1 = 0.0, u2 = 0.0;
+#pragma unroll
+      for (int k = 0; k < N; ++k) {
+        const double f = a.avf[line * N + k];
+        const int p = ROW ? m + N * k : k + N * m;
+        u1 = fma(f, rho[p * T + ci], u1);
+        u2 = fma(f, jz2[p * T + ci], u2);
+      }
+      y1[m] = u1;
+      y2[m] = u2;
+    }
+    double e1 = 0.0, e2 = 0.0;
+#pragma unroll
+    for (int m = 0; m < N; ++m) {
+      double h1 = 0.0, h2 = 0.0;
+#pragma unroll
+      for (int k = 0; k < N; ++k) {
+        h1 = fma(a.avf[m * N + k], y1[k], h1);
+        h2 = fma(a.avf[m * N + k], y2[k], h2);
+      }
+      const double d1 = h1 - rs[m], d2 = h2 - js2[m];
+      e1 += (d1 * d1) / (rs[m] * rs[m]);
+      e2 += (d2 * d2) / (js2[m] * js2[m]);
+    }
+    lineb[line * T + ci] = e1;
+    lineb[(N + line) * T + ci] = e2;
+  }
+  wave_lds_sync();
+  if (valid && line == 0) {
+    double s1 = lineb[ci], s2 = lineb[N * T + ci];
+#pragma unroll
+    for (int l = 1; l < N; ++l) {
+      s1 += lineb[l * T + ci];
+      s2 += lineb[(N + l) * T + ci];
+    }
+    const size_t mc = a.s2c[cell];
+    a.sm_ind[2 * mc] = log10(s1);
+    a.sm_ind[2 * mc + 1] = log10(s2);
+  }
 }
 
 // End face `rfp` (0 or N) of line `line` of family `fam`: own local face k
@@ -367,32 +435,21 @@ __global__ __launch_bounds__(STAGE_THREADS, 2) void stage_kernel(StageArgs a, Op
   const int line = (tid & 63) / T;     // N lines of T cells; the rest of the wave idles
   const int ci = (tid & 63) % T;
   const int cell = tile * T + ci;
-  // the tile's member (0 for one run) and valid cells (its first slots)
-  const int ti = a.tinfo[tile];
-  const int mem = ti >> 6, nvt = ti & 63;
-  const bool valid = line < N && ci < nvt;
-  const double ge = a.gmax[2 * mem], gu = a.gmax[2 * mem + 1];
-  const double cr = a.dtm ? a.cr * a.dtm[mem] : a.cr;   // b * dt of the member
-  const Phys &ph = a.phm[mem];                            // the member's free stream
-  const double c_h = sqrt(ge * (ge - gu));   // lagged GLM speed, compflux.f90:6
-  double eig = 0.0, um = 0.0, nres = 0.0, ndiv = 0.0;
-  const double *mt = a.met + (size_t)tile * G::NM * T + ci;
-  const int p0 = fam ? line : N * line;      // point of s = 0 on this line
-  const int ps = fam ? N : 1;                // point stride along the line
-  const int mx = fam ? G::S2X : G::S1X, my = fam ? G::S2Y : G::S1Y;
-  const double *ql = lds + LD::Q0 + p0 * T + ci;   // line values: ql[(v*N2 + ps*s)*T]
 
-  // ---- the face-table words of the line's two end faces first: the
-  //      neighbour gathers depend on them -----------------------------------------
-  int nb0 = 0, nbN = 0, info0 = 0, infoN = 0;
-  if (valid) {
-    face_words<N>(a, tile, ci, fam, line, 0, nb0, info0);
-    face_words<N>(a, tile, ci, fam, line, N, nbN, infoN);
-  }
-  // ---- the tile's state -> LDS in 16-byte loads (one contiguous block; all
-  //      loads are issued before the first LDS store), by the wave of the
-  //      line family with fewer neighbours in other tiles, so that the two
-  //      waves issue about as many vector loads before the barrier ----------
+  // ---- loads that depend on no other load, issued first: the face-table
+  //      words of the line's two end faces (the neighbour gathers depend on
+  //      them; every slot of a tile has them, so no lane waits for the
+  //      tile's valid-cell count first) and the tile's state, which one wave
+  //      stages into LDS in 16-byte loads (one contiguous block; all loads
+  //      are issued before the first LDS store), the wave of the line family
+  //      with fewer neighbours in other tiles, so that the two waves issue
+  //      about as many vector loads before the barrier ---------------------------
+  int nb0, nbN, info0, infoN;
+  face_words<N>(a, tile, ci, fam, line, 0, nb0, info0);
+  face_words<N>(a, tile, ci, fam, line, N, nbN, infoN);
+  // the tile's member (0 for one run) and valid cells (its first slots):
+  // loaded now, waited for only after the staging
+  const int ti = a.tinfo[tile];
   if (fam == a.sfam) {
     constexpr int NV2 = 9 * N2 * T / 2;                  // double2 per tile
     constexpr int NSW = (NV2 + 63) / 64;                 // rounds of one wave
@@ -412,6 +469,13 @@ __global__ __launch_bounds__(STAGE_THREADS, 2) void stage_kernel(StageArgs a, Op
     for (int k = 0; k < NSW - 1; ++k) dst[l + k * 64] = st[k];
     if (l + (NSW - 1) * 64 < NV2) dst[l + (NSW - 1) * 64] = st[NSW - 1];
   }
+  const int mem = ti >> 6, nvt = ti & 63;
+  const bool valid = line < N && ci < nvt;
+  const double *mt = a.met + (size_t)tile * G::NM * T + ci;
+  const int p0 = fam ? line : N * line;      // point of s = 0 on this line
+  const int ps = fam ? N : 1;                // point stride along the line
+  const int mx = fam ? G::S2X : G::S1X, my = fam ? G::S2Y : G::S1Y;
+  const double *ql = lds + LD::Q0 + p0 * T + ci;   // line values: ql[(v*N2 + ps*s)*T]
   // ---- neighbour traces at both end faces: gathered from memory for a
   //      neighbour in another tile, formed from the staged tile after the
   //      barrier for a neighbour in this one --------------------------------------
@@ -431,6 +495,13 @@ __global__ __launch_bounds__(STAGE_THREADS, 2) void stage_kernel(StageArgs a, Op
       msy[r] = mt[(my + r + NF * line) * T];
     }
   }
+  // the member's lagged maxima, step and free stream (needed from the face
+  // fluxes on)
+  const double ge = a.gmax[2 * mem], gu = a.gmax[2 * mem + 1];
+  const double cr = a.dtm ? a.cr * a.dtm[mem] : a.cr;   // b * dt of the member
+  const Phys &ph = a.phm[mem];                            // the member's free stream
+  const double c_h = sqrt(ge * (ge - gu));   // lagged GLM speed, compflux.f90:6
+  double eig = 0.0, um = 0.0, nres = 0.0, ndiv = 0.0;
   __syncthreads();
   if (valid && lci0 >= 0) face_trace_lds<N, double>(lds + LD::Q0, ops, info0, fam, line, 0, lci0, qn0);
   if (valid && lciN >= 0) face_trace_lds<N, double>(lds + LD::Q0, ops, infoN, fam, line, N, lciN, qnN);
@@ -561,13 +632,14 @@ __global__ __launch_bounds__(STAGE_THREADS, 2) void stage_kernel(StageArgs a, Op
   }
 
   // ---- exchange of the derivatives through LDS.  Round A: the eta-line
-  //      threads hand variables 0..VS-1 and the divB (Jz) weights to the
-  //      xi-line threads.  Round B: the xi-line threads hand variables VS..8 and
-  //      the summed divB weight to the eta-line threads.  Each LDS location is
-  //      read and then rewritten by the same xi-line lane, so one area serves
-  //      both rounds.  Afterwards D of each thread holds the sum of both
-  //      families for the variables it finishes, db the summed divB weight,
-  //      dj (xi-lines, AV) the summed Jz weight -------------------------------------
+  //      threads hand variables 0..VS-1 and the divB weights to the xi-line
+  //      threads.  Round B: the xi-line threads hand variables VS..8, the
+  //      summed divB weight and (AV) their Jz weights to the eta-line
+  //      threads.  Each LDS location is read and then rewritten by the same
+  //      xi-line lane, so one area serves both rounds.  Afterwards D of each
+  //      thread holds the sum of both families for the variables it
+  //      finishes, db the summed divB weight, dj (eta-lines, AV) the summed
+  //      Jz weight -----------------------------------------------------------------
   if (valid && fam == 1) {
 #pragma unroll
     for (int s = 0; s < N; ++s) {
@@ -575,7 +647,6 @@ __global__ __launch_bounds__(STAGE_THREADS, 2) void stage_kernel(StageArgs a, Op
 #pragma unroll
       for (int v = 0; v < VS; ++v) lds[LD::XD + (v * N2 + p) * T + ci] = D[s][v];
       lds[LD::XB + p * T + ci] = db[s];
-      if (AV) lds[LD::XJ + p * T + ci] = dj[s];
     }
   }
   __syncthreads();
@@ -586,10 +657,10 @@ __global__ __launch_bounds__(STAGE_THREADS, 2) void stage_kernel(StageArgs a, Op
 #pragma unroll
       for (int v = 0; v < VS; ++v) D[s][v] += lds[LD::XD + (v * N2 + p) * T + ci];
       db[s] += lds[LD::XB + p * T + ci];
-      if (AV) dj[s] += lds[LD::XJ + p * T + ci];
 #pragma unroll
       for (int v = VS; v < 9; ++v) lds[LD::XD + ((v - VS) * N2 + p) * T + ci] = D[s][v];
       lds[LD::XB + p * T + ci] = db[s];
+      if (AV) lds[LD::XJ + p * T + ci] = dj[s];
     }
   }
   __syncthreads();
@@ -600,6 +671,7 @@ __global__ __launch_bounds__(STAGE_THREADS, 2) void stage_kernel(StageArgs a, Op
 #pragma unroll
       for (int v = VS; v < 9; ++v) D[s][v] = lds[LD::XD + ((v - VS) * N2 + p) * T + ci] + D[s][v];
       db[s] = lds[LD::XB + p * T + ci];
+      if (AV) dj[s] += lds[LD::XJ + p * T + ci];
     }
   }
 
@@ -672,10 +744,13 @@ __global__ __launch_bounds__(STAGE_THREADS, 2) void stage_kernel(StageArgs a, Op
       if (V0 <= 6 && 6 < V1) R[6] -= divB * uy;
       if (V0 <= 7 && 7 < V1) R[7] -= divB * uz;
       if (V0 <= 8 && 8 < V1) R[8] -= ph.alpha * q[8];
-      if (AV && V0 == 0) {
+      if (AV && V0 == VS) {
+        // the point's Jz^2 replaces its Jz weight in LDS (this thread read
+        // that slot in the exchange; the sensor reads it back per cell)
         const double jz = dj[s] * ji;
         rho_s[s] = q[0];
         jz2_s[s] = jz * jz;
+        lds[LD::XJ + p * T + ci] = jz2_s[s];
       }
       if (NORMS) {
 #pragma unroll
@@ -728,6 +803,14 @@ __global__ __launch_bounds__(STAGE_THREADS, 2) void stage_kernel(StageArgs a, Op
     if (valid && fam == 1) compat_gradient(true);
   }
 
+  // ---- AV smoothness sensor (AV.f90:14-83), per cell, by the eta-line wave
+  //      (the one with fewer variables in the points step), before the
+  //      reduction barrier so that it overlaps the xi-line wave's points:
+  //      rho from the staged state, Jz^2 from the points step ------------------------
+  if (AV && fam == 1)
+    av_sensor_lines<N, T, false>(a, lds + LD::Q0, lds + LD::XJ, lds + LD::A_ROW, rho_s, jz2_s,
+                                 line, ci, cell, valid);
+
   // ---- block reductions: running maxima (getrusanov.f90:116-123,188-191) and
   //      norms (iterations.f90:37-46) ----------------------------------------------
   {
@@ -766,69 +849,6 @@ __global__ __launch_bounds__(STAGE_THREADS, 2) void stage_kernel(StageArgs a, Op
     }
   }
 
-  // ---- AV smoothness sensor (AV.f90:14-83), per cell --------------------------
-  if (AV) {
-    constexpr int NL = N - 1, NL2 = NL * NL;
-    const double *hlow = a.avtab;            // [isl][is]
-    const double *hhigh = a.avtab + NL * N;  // [is][isl]
-    double *rho = lds + LD::A_RHO;           // [p][ci]
-    double *jz2 = lds + LD::A_JZ;            // [p][ci]
-    double *lowb = lds + LD::A_LOW;          // [2][NL2][ci]
-    double *seb = lds + LD::A_SE;            // [2][N2][ci]
-    // the state area is free: every thread has read its points before the
-    // reduction barrier above
-    if (valid && fam == 0) {
-#pragma unroll
-      for (int s = 0; s < N; ++s) {
-        rho[(N * line + s) * T + ci] = rho_s[s];
-        jz2[(N * line + s) * T + ci] = jz2_s[s];
-      }
-    }
-    __syncthreads();
-    const int nvc = nvt;                      // valid cells of this tile
-    for (int it = tid; it < NL2 * T; it += STAGE_THREADS) {
-      const int c = it % T, t = it / T;
-      const int il = t % NL, jl = t / NL;
-      if (c >= nvc) continue;
-      double s1 = 0.0, s2 = 0.0;
-      for (int js = 0; js < N; ++js)
-        for (int is = 0; is < N; ++is) {
-          const double h = hlow[il * N + is] * hlow[jl * N + js];
-          s1 = fma(rho[(is + N * js) * T + c], h, s1);
-          s2 = fma(jz2[(is + N * js) * T + c], h, s2);
-        }
-      lowb[(0 * NL2 + t) * T + c] = s1;
-      lowb[(1 * NL2 + t) * T + c] = s2;
-    }
-    __syncthreads();
-    for (int it = tid; it < N2 * T; it += STAGE_THREADS) {
-      const int c = it % T, p = it / T;
-      const int is = p % N, js = p / N;
-      if (c >= nvc) continue;
-      double h1 = 0.0, h2 = 0.0;
-      for (int jl = 0; jl < NL; ++jl)
-        for (int il = 0; il < NL; ++il) {
-          const double h = hhigh[is * NL + il] * hhigh[js * NL + jl];
-          h1 = fma(lowb[(0 * NL2 + il + NL * jl) * T + c], h, h1);
-          h2 = fma(lowb[(1 * NL2 + il + NL * jl) * T + c], h, h2);
-        }
-      const double r0 = rho[p * T + c], j0 = jz2[p * T + c];
-      const double d1 = h1 - r0, d2 = h2 - j0;
-      seb[(0 * N2 + p) * T + c] = (d1 * d1) / (r0 * r0);
-      seb[(1 * N2 + p) * T + c] = (d2 * d2) / (j0 * j0);
-    }
-    __syncthreads();
-    for (int c = tid; c < T && c < nvc; c += STAGE_THREADS) {
-      double s1 = 0.0, s2 = 0.0;
-      for (int p = 0; p < N2; ++p) {
-        s1 += seb[(0 * N2 + p) * T + c];
-        s2 += seb[(1 * N2 + p) * T + c];
-      }
-      const size_t mc = a.s2c[tile * T + c];
-      a.sm_ind[2 * mc] = log10(s1);
-      a.sm_ind[2 * mc + 1] = log10(s2);
-    }
-  }
 }
 
 }  // namespace sd2d

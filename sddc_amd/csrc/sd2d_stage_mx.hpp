@@ -44,15 +44,14 @@ template <int N, int MODE> struct StageLdsMx {
   static constexpr int XB = XD + XV * N2 * T;
   static constexpr int XJ = XB + N2 * T;
   static constexpr int FLOATS = XJ + (AV ? N2 * T : 0);
-  // in doubles, after the float area: reduction slots, and the AV sensor
-  // scratch (rho, Jz^2, low-order projections, point errors)
+  // in doubles, after the float area: reduction slots, and the AV sensor's
+  // FP64 rho and Jz^2 [p][ci] and row sums [2][line][ci]
   static constexpr int D0 = (FLOATS + 1) / 2;
   static constexpr int RED = D0;
   static constexpr int A_RHO = RED + 4 * (STAGE_THREADS / 64);
   static constexpr int A_JZ = A_RHO + N2 * T;
-  static constexpr int A_LOW = A_JZ + N2 * T;
-  static constexpr int A_SE = A_LOW + 2 * NL * NL * T;
-  static constexpr int TOTAL_D = AV ? A_SE + 2 * N2 * T : A_RHO;
+  static constexpr int A_ROW = A_JZ + N2 * T;
+  static constexpr int TOTAL_D = AV ? A_ROW + 2 * N * T : A_RHO;
 };
 
 template <int N, int MODE>
@@ -84,32 +83,16 @@ void stage_kernel_mx(StageArgs a, OpsF ops, const float *__restrict__ metf) {
   const int fam = tid >> 6;            // 0: xi-line (row js), 1: eta-line (column is)
   const int line = (tid & 63) / T;     // N lines of T cells; the rest of the wave idles
   const int ci = (tid & 63) % T;
+  // ---- loads that depend on no other load, issued first (as in
+  //      stage_kernel): the face-table words of the line's two end faces,
+  //      the tile's info (waited for only after the staging), and the
+  //      tile's state -> LDS as FP32, by the wave of the line family with
+  //      fewer neighbours in other tiles ----------------------------------------------
+  int nb0, nbN, info0, infoN;
+  face_words<N>(a, tile, ci, fam, line, 0, nb0, info0);
+  face_words<N>(a, tile, ci, fam, line, N, nbN, infoN);
   // the tile's member (0 for one run) and valid cells (its first slots)
   const int ti = a.tinfo[tile];
-  const int mem = ti >> 6, nvt = ti & 63;
-  const bool valid = line < N && ci < nvt;
-  const double ge = a.gmax[2 * mem], gu = a.gmax[2 * mem + 1];
-  const double cr = a.dtm ? a.cr * a.dtm[mem] : a.cr;   // b * dt of the member
-  const Phys &ph = a.phm[mem];                            // the member's free stream
-  const float c_h = float(sqrt(ge * (ge - gu)));   // compflux.f90:6
-  float eig = 0.f, um = 0.f;
-  double nres = 0.0, ndiv = 0.0;
-  const float *mtf = metf + (size_t)tile * G::NM * T + ci;
-  const double *mt = a.met + (size_t)tile * G::NM * T + ci;
-  const int p0 = fam ? line : N * line;
-  const int ps = fam ? N : 1;
-  const int mx = fam ? G::S2X : G::S1X, my = fam ? G::S2Y : G::S1Y;
-  const float *ql = ldf + LD::QF + p0 * T + ci;
-
-  // ---- the face-table words of the line's two end faces first: the
-  //      neighbour gathers depend on them -----------------------------------------
-  int nb0 = 0, nbN = 0, info0 = 0, infoN = 0;
-  if (valid) {
-    face_words<N>(a, tile, ci, fam, line, 0, nb0, info0);
-    face_words<N>(a, tile, ci, fam, line, N, nbN, infoN);
-  }
-  // ---- the tile's state -> LDS as FP32, by the wave of the line family
-  //      with fewer neighbours in other tiles (as in stage_kernel) ------------
   if (fam == a.sfam) {
     constexpr int NV2 = 9 * N2 * T / 2;
     constexpr int NSW = (NV2 + 63) / 64;
@@ -128,6 +111,14 @@ void stage_kernel_mx(StageArgs a, OpsF ops, const float *__restrict__ metf) {
       if (k < NSW - 1 || i < NV2) dst[i] = make_float2(float(st[k].x), float(st[k].y));
     }
   }
+  const int mem = ti >> 6, nvt = ti & 63;
+  const bool valid = line < N && ci < nvt;
+  const float *mtf = metf + (size_t)tile * G::NM * T + ci;
+  const double *mt = a.met + (size_t)tile * G::NM * T + ci;
+  const int p0 = fam ? line : N * line;
+  const int ps = fam ? N : 1;
+  const int mx = fam ? G::S2X : G::S1X, my = fam ? G::S2Y : G::S1Y;
+  const float *ql = ldf + LD::QF + p0 * T + ci;
   // ---- neighbour traces at both end faces (memory or the staged tile) -------
   float qn0[9], qnN[9];
   int lci0 = -1, lciN = -1;
@@ -145,6 +136,14 @@ void stage_kernel_mx(StageArgs a, OpsF ops, const float *__restrict__ metf) {
       msy[r] = mtf[(my + r + NF * line) * T];
     }
   }
+  // the member's lagged maxima, step and free stream (needed from the face
+  // fluxes on)
+  const double ge = a.gmax[2 * mem], gu = a.gmax[2 * mem + 1];
+  const double cr = a.dtm ? a.cr * a.dtm[mem] : a.cr;   // b * dt of the member
+  const Phys &ph = a.phm[mem];                            // the member's free stream
+  const float c_h = float(sqrt(ge * (ge - gu)));   // compflux.f90:6
+  float eig = 0.f, um = 0.f;
+  double nres = 0.0, ndiv = 0.0;
   __syncthreads();
   if (valid && lci0 >= 0) face_trace_lds<N, float>(ldf + LD::QF, ops, info0, fam, line, 0, lci0, qn0);
   if (valid && lciN >= 0) face_trace_lds<N, float>(ldf + LD::QF, ops, infoN, fam, line, N, lciN, qnN);
@@ -336,6 +335,8 @@ void stage_kernel_mx(StageArgs a, OpsF ops, const float *__restrict__ metf) {
         const double jz = double(dj[s]) * ji;
         rho_s[s] = q[0];
         jz2_s[s] = jz * jz;
+        lds[LD::A_RHO + p * T + ci] = rho_s[s];
+        lds[LD::A_JZ + p * T + ci] = jz2_s[s];
       }
       if (NORMS) {
 #pragma unroll
@@ -412,65 +413,11 @@ void stage_kernel_mx(StageArgs a, OpsF ops, const float *__restrict__ metf) {
     }
   }
 
-  // ---- AV smoothness sensor (AV.f90:14-83), in FP64 ---------------------------
-  if (AV) {
-    constexpr int NL = N - 1, NL2 = NL * NL;
-    const double *hlow = a.avtab;
-    const double *hhigh = a.avtab + NL * N;
-    double *rho = lds + LD::A_RHO, *jz2 = lds + LD::A_JZ;
-    double *lowb = lds + LD::A_LOW, *seb = lds + LD::A_SE;
-    if (valid && fam == 0) {
-#pragma unroll
-      for (int s = 0; s < N; ++s) {
-        rho[(N * line + s) * T + ci] = rho_s[s];
-        jz2[(N * line + s) * T + ci] = jz2_s[s];
-      }
-    }
-    __syncthreads();
-    const int nvc = nvt;                      // valid cells of this tile
-    for (int it = tid; it < NL2 * T; it += STAGE_THREADS) {
-      const int c = it % T, t = it / T;
-      const int il = t % NL, jl = t / NL;
-      if (c >= nvc) continue;
-      double s1 = 0.0, s2 = 0.0;
-      for (int js = 0; js < N; ++js)
-        for (int is = 0; is < N; ++is) {
-          const double h = hlow[il * N + is] * hlow[jl * N + js];
-          s1 = fma(rho[(is + N * js) * T + c], h, s1);
-          s2 = fma(jz2[(is + N * js) * T + c], h, s2);
-        }
-      lowb[(0 * NL2 + t) * T + c] = s1;
-      lowb[(1 * NL2 + t) * T + c] = s2;
-    }
-    __syncthreads();
-    for (int it = tid; it < N2 * T; it += STAGE_THREADS) {
-      const int c = it % T, p = it / T;
-      const int is = p % N, js = p / N;
-      if (c >= nvc) continue;
-      double h1 = 0.0, h2 = 0.0;
-      for (int jl = 0; jl < NL; ++jl)
-        for (int il = 0; il < NL; ++il) {
-          const double h = hhigh[is * NL + il] * hhigh[js * NL + jl];
-          h1 = fma(lowb[(0 * NL2 + il + NL * jl) * T + c], h, h1);
-          h2 = fma(lowb[(1 * NL2 + il + NL * jl) * T + c], h, h2);
-        }
-      const double r0 = rho[p * T + c], j0 = jz2[p * T + c];
-      const double d1 = h1 - r0, d2 = h2 - j0;
-      seb[(0 * N2 + p) * T + c] = (d1 * d1) / (r0 * r0);
-      seb[(1 * N2 + p) * T + c] = (d2 * d2) / (j0 * j0);
-    }
-    __syncthreads();
-    for (int c = tid; c < T && c < nvc; c += STAGE_THREADS) {
-      double s1 = 0.0, s2 = 0.0;
-      for (int p = 0; p < N2; ++p) {
-        s1 += seb[(0 * N2 + p) * T + c];
-        s2 += seb[(1 * N2 + p) * T + c];
-      }
-      const size_t mc = a.s2c[tile * T + c];
-      a.sm_ind[2 * mc] = log10(s1);
-      a.sm_ind[2 * mc + 1] = log10(s2);
-    }
-  }
+  // ---- AV smoothness sensor (AV.f90:14-83), in FP64, per cell, by the
+  //      xi-line wave from the FP64 rho and Jz^2 of the points step -----------------
+  if (AV && fam == 0)
+    av_sensor_lines<N, T, true>(a, lds + LD::A_RHO, lds + LD::A_JZ, lds + LD::A_ROW, rho_s, jz2_s, line,
+                         ci, tile * T + ci, valid);
 }
 
 }  // namespace sd2d

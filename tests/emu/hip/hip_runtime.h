@@ -8,7 +8,9 @@
 // Execution model of a launch (tests/emu/emu_runtime.hpp): workgroups run
 // one after another; the 128 work-items of a workgroup are 128 host
 // threads.  __syncthreads is a barrier over the workgroup's live threads,
-// __shfl_xor an exchange among the 64 lanes of a wavefront, atomicMax a
+// __shfl_xor an exchange among the 64 lanes of a wavefront (and
+// __builtin_amdgcn_wave_barrier a barrier over them, since host threads do
+// not run in lockstep as a wavefront's lanes do), atomicMax a
 // locked read-modify-write.  Dynamic LDS is one buffer shared by the
 // running workgroup, poisoned with NaN before every workgroup so that a
 // read of LDS nothing wrote shows up in the results.
@@ -26,6 +28,8 @@
 #define __shared__
 #define __launch_bounds__(...)
 #define __builtin_amdgcn_sched_barrier(x) ((void)0)
+#define __builtin_amdgcn_fence(order, scope) ((void)0)
+#define __builtin_amdgcn_wave_barrier() emu::wave_sync()
 #define __builtin_amdgcn_rcp(x) (1.0 / (x))
 #define __builtin_amdgcn_rcpf(x) (1.0f / (x))
 
@@ -36,6 +40,7 @@ struct emu_dim3 {
 namespace emu {
 extern thread_local emu_dim3 tid3, bid3;
 void syncthreads();
+void wave_sync();
 uint64_t shfl_xor_bits(uint64_t bits, int mask);
 unsigned long long atomic_max(unsigned long long *p, unsigned long long v);
 }  // namespace emu

@@ -91,6 +91,11 @@ void syncthreads() {
   wait_group(g_bs->blk, lk, "__syncthreads");
 }
 
+void wave_sync() {
+  std::unique_lock<std::mutex> lk(g_bs->m);
+  wait_group(g_bs->wave[(int)tid3.x >> 6], lk, "a wave-level LDS sync");
+}
+
 uint64_t shfl_xor_bits(uint64_t b, int mask) {
   const int t = (int)tid3.x, w = t >> 6, l = t & 63;
   if (mask < 0 || mask > 63) die("__shfl_xor lane mask outside the wavefront");
@@ -173,7 +178,7 @@ std::vector<T> dev_buffer(size_t n) {   // uninitialised device memory: NaN
 struct Ctx {
   int N = 0, nc = 0, nt = 0, cpw = 0, precision = 0;
   HostTables tb;
-  std::vector<double> q[3], fsr[3], met, gmax, sm_ind, normbuf, resid_f, nabla_f, avtab;
+  std::vector<double> q[3], fsr[3], met, gmax, sm_ind, normbuf, resid_f, nabla_f;
   std::vector<int> ftab, tinfo, s2c;
   std::vector<unsigned long long> slots;
   std::vector<float> metf;
@@ -246,7 +251,8 @@ StageArgs base_args(Ctx &c) {
   a.ftab = c.ftab.data();
   a.gmax = c.gmax.data();
   a.slots = c.slots.data();
-  a.avtab = c.avtab.data();
+  if (c.tb.av.size() != (size_t)c.N * c.N) emu::die("AV filter of the wrong size");
+  std::copy(c.tb.av.begin(), c.tb.av.end(), a.avf);
   a.ncell = c.nc;
   a.ntile = c.nt;
   a.sm_ind = c.sm_ind.data();
@@ -332,7 +338,6 @@ int main(int argc, char **argv) {
   c.ftab = c.tb.ft;
   c.tinfo = c.tb.tvalid;   // one run: member 0, the tile's valid cells
   c.s2c = c.tb.s2c;
-  c.avtab = c.tb.av;
   c.gmax = dev_buffer<double>(2);
   c.slots = dev_buffer<unsigned long long>(2 * (size_t)NSLOT);
   c.sm_ind = dev_buffer<double>(2 * (size_t)nc);
