@@ -164,8 +164,8 @@ __device__ __forceinline__ double wave_sum(double x) {
 // points in line order; only lanes of this wave exchange values, so the
 // caller needs no workgroup barrier but must call this from every lane of
 // the wave (a wave-level LDS sync inside).  The sums run in another order
-// than the reference's, so values differ from it in the last bits of the
-// smoothed field.
+// than the reference's and rho's quotient uses the reciprocal, so values
+// differ from it in the last bits of the smoothed field.
 template <int N, int T, bool ROW>
 __device__ __forceinline__ void av_sensor_lines(const StageArgs &a, const double *rho,
                                                 const double *jz2, double *lineb,
@@ -196,24 +196,25 @@ __device__ __forceinline__ void av_sensor_lines(const StageArgs &a, const double
         h1 = fma(a.avf[m * N + k], y1[k], h1);
         h2 = fma(a.avf[m * N + k], y2[k], h2);
       }
-      const double d1 = h1 - rs[m], d2 = h2 - js2[m];
-      e1 += (d1 * d1) / (rs[m] * rs[m]);
+      // rho > 0: its quotient through the reciprocal (within a few ulp);
+      // Jz^2 may be 0, so its quotient keeps IEEE division (inf or NaN as
+      // in the reference)
+      const double d1 = (h1 - rs[m]) * rcp(rs[m]), d2 = h2 - js2[m];
+      e1 = fma(d1, d1, e1);
       e2 += (d2 * d2) / (js2[m] * js2[m]);
     }
     lineb[line * T + ci] = e1;
     lineb[(N + line) * T + ci] = e2;
   }
   wave_lds_sync();
-  if (valid && line == 0) {
-    double s1 = lineb[ci], s2 = lineb[N * T + ci];
+  // lane line 0 of a cell finishes rho's sum, lane line 1 Jz's, so the
+  // wave evaluates log10 once
+  if (valid && line < 2) {
+    const double *sb = lineb + line * N * T + ci;
+    double sum = sb[0];
 #pragma unroll
-    for (int l = 1; l < N; ++l) {
-      s1 += lineb[l * T + ci];
-      s2 += lineb[(N + l) * T + ci];
-    }
-    const size_t mc = a.s2c[cell];
-    a.sm_ind[2 * mc] = log10(s1);
-    a.sm_ind[2 * mc + 1] = log10(s2);
+    for (int l = 1; l < N; ++l) sum += sb[l * T];
+    a.sm_ind[2 * (size_t)a.s2c[cell] + line] = log10(sum);
   }
 }
 
