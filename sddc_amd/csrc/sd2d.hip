@@ -150,20 +150,18 @@ static int fail(sd2d_ctx *c, int code, const std::string &m) {
 // ---------------------------------------------------------------------------
 // small device kernels
 // ---------------------------------------------------------------------------
-// Fortran Q(9,N,N,NCELL) -> tile-major SoA through the slot -> cell map
+// Fortran Q(9,N,N,NCELL) -> tile layout (qo) through the slot -> cell map
 // (padding slots get zeros; no valid lane reads them).
 template <int N>
 __global__ void k_f2t(const double *__restrict__ f, double *__restrict__ t,
                       const int *__restrict__ s2c, int ntile) {
-  constexpr int CPW = Geo<N>::CPW, N2 = Geo<N>::N2;
-  const size_t tot = (size_t)ntile * CPW * 9 * N2;
+  constexpr int CPW = Geo<N>::CPW, N2 = Geo<N>::N2, QT = Geo<N>::QT;
+  const size_t tot = (size_t)ntile * QT;
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < tot;
        i += (size_t)gridDim.x * blockDim.x) {
-    const int ci = int(i % CPW);
-    const size_t r = i / CPW;
-    const int vp = int(r % (9 * N2));
-    const size_t tile = r / (9 * N2);
-    const int v = vp / N2, p = vp % N2;
+    const size_t tile = i / QT;
+    int v, p, ci;
+    qo_inv<N>(int(i - tile * QT), v, p, ci);
     const int cell = s2c[tile * CPW + ci];
     t[i] = cell >= 0 ? f[v + 9 * (p + (size_t)N2 * cell)] : 0.0;
   }
@@ -180,8 +178,8 @@ __global__ void k_t2f(const double *__restrict__ t, double *__restrict__ f,
     const size_t r = i / 9;
     const int p = int(r % N2);
     const size_t slot = c2s[r / N2];
-    const size_t tile = slot / CPW, ci = slot % CPW;
-    f[i] = t[(tile * 9 * N2 + v * N2 + p) * CPW + ci];
+    const size_t tile = slot / CPW;
+    f[i] = t[tile * Geo<N>::QT + qo<N>(v, p, int(slot % CPW))];
   }
 }
 
@@ -285,7 +283,8 @@ __global__ void k_plot_nodes(const double *__restrict__ q, const int *__restrict
     const int cell = code / (np1 * np1), loc = code % (np1 * np1);
     const int ii = loc % np1, jj = loc / np1;
     const int sl = c2s[cell];
-    const double *b = q + (size_t)(sl / T) * QT + sl % T;
+    const double *b = q + (size_t)(sl / T) * QT;
+    const int sci = sl % T;
     double qs[9];
 #pragma unroll
     for (int v = 0; v < 9; ++v) qs[v] = 0.0;
@@ -293,7 +292,7 @@ __global__ void k_plot_nodes(const double *__restrict__ q, const int *__restrict
       for (int is = 0; is < N; ++is) {
         const double hi = h[ii * N + is], hj = h[jj * N + js];
 #pragma unroll
-        for (int v = 0; v < 9; ++v) qs[v] = qs[v] + (b[(v * N2 + is + N * js) * T] * hi) * hj;
+        for (int v = 0; v < 9; ++v) qs[v] = qs[v] + (b[qo<N>(v, is + N * js, sci)] * hi) * hj;
       }
 #pragma unroll
     for (int v = 0; v < 9; ++v) acc[v] = acc[v] + qs[v];

@@ -94,12 +94,12 @@ __global__ __launch_bounds__(STAGE_THREADS) void adj_point_kernel(AdjArgs a) {
     if (cell < 0) continue;   // padding slot
     const int is = p % N, js = p / N;
     const double ji = mt[(G::JI + js + N * is) * T + ci];
-    const size_t off = (size_t)tile * G::QT + ci;
+    const size_t off = (size_t)tile * G::QT;
     double q[9], rb[9];
 #pragma unroll
     for (int v = 0; v < 9; ++v) {
-      q[v] = a.q[off + (size_t)(v * N2 + p) * T];
-      rb[v] = a.rscale * a.rbar[off + (size_t)(v * N2 + p) * T];
+      q[v] = a.q[off + qo<N>(v, p, ci)];
+      rb[v] = a.rscale * a.rbar[off + qo<N>(v, p, ci)];
     }
     const double divB = a.divb[(size_t)tile * N2 * T + (size_t)p * T + ci];
     const double ri = 1.0 / q[0];
@@ -126,7 +126,7 @@ __global__ __launch_bounds__(STAGE_THREADS) void adj_point_kernel(AdjArgs a) {
     qb[0] = -(ubx * ux + uby * uy + ubz * uz) * ri;
     qb[8] = -a.alpha * rb[8];
 #pragma unroll
-    for (int v = 0; v < 9; ++v) a.qbar[off + (size_t)(v * N2 + p) * T] += qb[v];
+    for (int v = 0; v < 9; ++v) a.qbar[off + qo<N>(v, p, ci)] += qb[v];
   }
 }
 
@@ -145,15 +145,20 @@ __device__ __forceinline__ void adj_flux_cot(const double *gb, int p0, int ps, i
   }
 }
 
+// Trace of the 9 variables along the line through point p0 (stride ps) of
+// the slot `base` = tile * QT + ci (the face-table encoding) of the
+// state-layout array q.
 template <int N>
-__device__ __forceinline__ void adj_trace(const double *qb, int p0, int ps, const double *Lr,
-                                          double *t) {
-  constexpr int T = Geo<N>::CPW, N2 = Geo<N>::N2;
+__device__ __forceinline__ void adj_trace(const double *q, int base, int p0, int ps,
+                                          const double *Lr, double *t) {
+  constexpr int QT = Geo<N>::QT;
+  const int nt = base / QT, ci = base - nt * QT;
+  const double *qt = q + (size_t)nt * QT;
 #pragma unroll
   for (int v = 0; v < 9; ++v) {
     double vals[N];
 #pragma unroll
-    for (int s = 0; s < N; ++s) vals[s] = qb[(size_t)(v * N2 + p0 + ps * s) * T];
+    for (int s = 0; s < N; ++s) vals[s] = qt[qo<N>(v, p0 + ps * s, ci)];
     t[v] = line_trace<N, double>(vals, Lr);
   }
 }
@@ -342,7 +347,7 @@ __device__ void adj_point_flux(const AdjArgs &a, const Ops &ops, double c_h, int
 #pragma unroll
   for (int s = 0; s < N; ++s) Lr[s] = ops.L[r][s];
   double t[9], Fb[9], Wb;
-  adj_trace<N>(a.q + (size_t)tile * QT + ci, p0, ps, Lr, t);
+  adj_trace<N>(a.q, tile * QT + ci, p0, ps, Lr, t);
   adj_flux_cot<N>(a.g + (size_t)tile * 10 * N2 * T + ci, p0, ps, r, ops, Fb, Wb);
   if (r != 0 && r != N) {
     // interior flux point (compflux.f90:53-104) and the divB weight of the trace
@@ -369,7 +374,7 @@ __device__ void adj_point_flux(const AdjArgs &a, const Ops &ops, double c_h, int
 #pragma unroll
     for (int s = 0; s < N; ++s) Ln[s] = endN ? ops.L[N][s] : ops.L[0][s];
     double qn[9], Fnb[9], Wnb;
-    adj_trace<N>(a.q + nbase, np0, nst, Ln, qn);
+    adj_trace<N>(a.q, nbase, np0, nst, Ln, qn);
     adj_flux_cot<N>(a.g + (size_t)nt * 10 * N2 * T + nci, np0, nst, endN ? N : 0, ops, Fnb, Wnb);
     const int sl = left ? own_sl : ((kn == 0 || kn == 3) ? -1 : 1);
     const int sr = left ? ((kn == 1 || kn == 2) ? -1 : 1) : ((k == 1 || k == 2) ? -1 : 1);
@@ -469,7 +474,7 @@ __global__ __launch_bounds__(STAGE_THREADS) void adj_line_kernel(AdjArgs a, Ops 
   }
   if (!valid || fam != 0) return;
   const int js = line;
-  double *qb = a.qbar + (size_t)tile * G::QT + ci;
+  double *qb = a.qbar + (size_t)tile * G::QT;
 #pragma unroll
   for (int is = 0; is < N; ++is)
 #pragma unroll
@@ -479,7 +484,7 @@ __global__ __launch_bounds__(STAGE_THREADS) void adj_line_kernel(AdjArgs a, Ops 
       for (int r = 0; r < NF; ++r) acc += ops.L[r][is] * lds[LD::at(0, js, r, v, ci)];
 #pragma unroll
       for (int r = 0; r < NF; ++r) acc += ops.L[r][js] * lds[LD::at(1, is, r, v, ci)];
-      qb[(size_t)(v * N2 + is + N * js) * T] += acc;
+      qb[qo<N>(v, is + N * js, ci)] += acc;
     }
 }
 
@@ -504,7 +509,7 @@ __global__ void drag_kernel(const double *q, const int *ftab, const double *met,
   const int tile = slot / T, ci = slot % T;
   double d = 0.0;
   {   // (a padding slot has no wall face: d stays 0)
-    const double *qc = q + (size_t)tile * G::QT + ci;
+    const double *qc = q + (size_t)tile * G::QT;
     const double *mt = met + (size_t)tile * G::NM * T + ci;
     for (int k = 0; k < 4; ++k) {
       const int info = ftab[((size_t)tile * 8 + k * 2 + 1) * T + ci];
@@ -519,7 +524,7 @@ __global__ void drag_kernel(const double *q, const int *ftab, const double *met,
         double t[9];
         for (int v = 0; v < 9; ++v) {
           double acc = 0.0;
-          for (int s = 0; s < N; ++s) acc = fma(qc[(size_t)(v * N2 + p0 + ps * s) * T], ops.L[rfp][s], acc);
+          for (int s = 0; s < N; ++s) acc = fma(qc[qo<N>(v, p0 + ps * s, ci)], ops.L[rfp][s], acc);
           t[v] = acc;
         }
         const double g1 = GAM - 1.0;
@@ -540,9 +545,9 @@ __global__ void drag_kernel(const double *q, const int *ftab, const double *met,
           pt[6] = -c * t[6];
           pt[7] = -c * t[7];
           pt[8] = -c * t[8];
-          double *qb = qbar + (size_t)tile * G::QT + ci;
+          double *qb = qbar + (size_t)tile * G::QT;
           for (int v = 0; v < 9; ++v)
-            for (int s = 0; s < N; ++s) qb[(size_t)(v * N2 + p0 + ps * s) * T] += pt[v] * ops.L[rfp][s];
+            for (int s = 0; s < N; ++s) qb[qo<N>(v, p0 + ps * s, ci)] += pt[v] * ops.L[rfp][s];
         }
       }
     }
@@ -639,11 +644,11 @@ __global__ __launch_bounds__(STAGE_THREADS) void adj_argmax_kernel(AdjArgs a, Op
     const bool face = r == 0 || r == N;
     double Lr[N], t[9], qn[9];
     for (int s = 0; s < N; ++s) Lr[s] = ops.L[r][s];
-    adj_trace<N>(a.q + (size_t)tile * QT + ci, p0, ps, Lr, t);
+    adj_trace<N>(a.q, tile * QT + ci, p0, ps, Lr, t);
     if (face && ((info >> 3) & 7) == FT_INTERIOR) {
       double Ln[N];
       for (int s = 0; s < N; ++s) Ln[s] = endN ? ops.L[N][s] : ops.L[0][s];
-      adj_trace<N>(a.q + nbase, np0, nst, Ln, qn);
+      adj_trace<N>(a.q, nbase, np0, nst, Ln, qn);
     }
     double e, u;
     adj_point_speeds<N, double>(a, info, k, r, face, sx, sy, t, qn, e, u);
@@ -685,10 +690,11 @@ __global__ void adj_argmax_apply(AdjArgs a, Ops ops_arg, int loc, int kind, doub
     Lr[s] = ops.L[r][s];
     Ln[s] = endN ? ops.L[N][s] : ops.L[0][s];
   }
-  adj_trace<N>(a.q + (size_t)tile * QT + ci, p0, ps, Lr, t);
-  if (nbr) adj_trace<N>(a.q + nbase, np0, nst, Ln, qn);
-  double *qo = a.qbar + (size_t)tile * QT + ci;
-  double *qnb = a.qbar + nbase;
+  adj_trace<N>(a.q, tile * QT + ci, p0, ps, Lr, t);
+  if (nbr) adj_trace<N>(a.q, nbase, np0, nst, Ln, qn);
+  double *qob = a.qbar + (size_t)tile * QT;
+  const int nbt = nbase / QT, nbc = nbase - nbt * QT;
+  double *qnb = a.qbar + (size_t)nbt * QT;
   for (int kk = 0; kk < (nbr ? 18 : 9); ++kk) {
     Dual td[9], nd[9], e, u;
     for (int v = 0; v < 9; ++v) {
@@ -698,9 +704,9 @@ __global__ void adj_argmax_apply(AdjArgs a, Ops ops_arg, int loc, int kind, doub
     adj_point_speeds<N, Dual>(a, info, k, r, face, sx, sy, td, nd, e, u);
     const double d = w * (kind == 0 ? e.d : u.d);
     if (kk < 9) {
-      for (int s = 0; s < N; ++s) qo[(size_t)(kk * N2 + p0 + ps * s) * T] += d * Lr[s];
+      for (int s = 0; s < N; ++s) qob[qo<N>(kk, p0 + ps * s, ci)] += d * Lr[s];
     } else {
-      for (int s = 0; s < N; ++s) qnb[(size_t)((kk - 9) * N2 + np0 + nst * s) * T] += d * Ln[s];
+      for (int s = 0; s < N; ++s) qnb[qo<N>(kk - 9, np0 + nst * s, nbc)] += d * Ln[s];
     }
   }
 }

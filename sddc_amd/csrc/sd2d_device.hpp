@@ -36,6 +36,40 @@ template <int N> struct Geo {
   static constexpr int QT = 9 * N2 * CPW;   // doubles per state tile
 };
 
+// Offset of variable v at solution point p (= is + N js) of slot ci inside a
+// state tile of T = CPW cells (9 N^2 T doubles, in HBM and staged in LDS
+// alike): variables 0-7 in pairs, [v/2][p][ci][v%2], so that one 16-byte
+// access moves two variables of a point of a cell, then variable 8 alone,
+// [p][ci].  Every kernel that reads or writes a state-shaped tile array
+// (state, RK registers, viscous divergence, adjoint cotangents) indexes it
+// through qo, and qo_inv is its inverse.
+template <int N>
+__host__ __device__ constexpr int qo(int v, int p, int ci) {
+  constexpr int T = cpw_of(N), N2 = N * N;
+  return v < 8 ? (((v >> 1) * N2 + p) * T + ci) * 2 + (v & 1) : 8 * N2 * T + p * T + ci;
+}
+template <int N>
+__host__ __device__ inline void qo_inv(int off, int &v, int &p, int &ci) {
+  constexpr int T = cpw_of(N), N2 = N * N;
+  if (off < 8 * N2 * T) {
+    const int k = off / (2 * N2 * T), r = off - k * 2 * N2 * T;
+    p = r / (2 * T);
+    const int r2 = r - p * 2 * T;
+    ci = r2 >> 1;
+    v = 2 * k + (r2 & 1);
+  } else {
+    const int r = off - 8 * N2 * T;
+    p = r / T;
+    ci = r - p * T;
+    v = 8;
+  }
+}
+// the same offset for an order known only at run time (host code)
+inline int qo_rt(int N, int v, int p, int ci) {
+  const int T = cpw_of(N), N2 = N * N;
+  return v < 8 ? (((v >> 1) * N2 + p) * T + ci) * 2 + (v & 1) : 8 * N2 * T + p * T + ci;
+}
+
 // Values handed between lanes of one wavefront through LDS: the LDS unit
 // performs a wavefront's operations in order, so no wait is needed between
 // one lane's store and another lane's load; the fences keep the compiler

@@ -160,13 +160,14 @@ __device__ __forceinline__ double wave_sum(double x) {
 // of column is = line), then along the line, and sums its points' terms
 // (smoothed - x)^2 / x^2; the N line sums of a cell meet in LDS and lane
 // line 0 of the cell takes log10 of each total.  rho and jz2 hold the
-// cell's values [p][ci] (p = is + N js) in LDS, rs and js2 the thread's own
+// cell's values in LDS at [(p T + ci) RS] and [p T + ci] (p = is + N js;
+// RS = 2 for rho read from the staged tile's first variable pair), rs and js2 the thread's own
 // points in line order; only lanes of this wave exchange values, so the
 // caller needs no workgroup barrier but must call this from every lane of
 // the wave (a wave-level LDS sync inside).  The sums run in another order
 // than the reference's and rho's quotient uses the reciprocal, so values
 // differ from it in the last bits of the smoothed field.
-template <int N, int T, bool ROW>
+template <int N, int T, bool ROW, int RS>
 __device__ __forceinline__ void av_sensor_lines(const StageArgs &a, const double *rho,
                                                 const double *jz2, double *lineb,
                                                 const double (&rs)[N], const double (&js2)[N],
@@ -181,7 +182,7 @@ __device__ __forceinline__ void av_sensor_lines(const StageArgs &a, const double
       for (int k = 0; k < N; ++k) {
         const double f = a.avf[line * N + k];
         const int p = ROW ? m + N * k : k + N * m;
-        u1 = fma(f, rho[p * T + ci], u1);
+        u1 = fma(f, rho[(p * T + ci) * RS], u1);
         u2 = fma(f, jz2[p * T + ci], u2);
       }
       y1[m] = u1;
@@ -215,6 +216,62 @@ __device__ __forceinline__ void av_sensor_lines(const StageArgs &a, const double
 #pragma unroll
     for (int l = 1; l < N; ++l) sum += sb[l * T];
     a.sm_ind[2 * (size_t)a.s2c[cell] + line] = log10(sum);
+  }
+}
+
+// The values of all 9 variables along a line of a state tile q (in HBM or
+// staged in LDS, FP64 or FP32): f(v, vals) is called for v = 0..8 with
+// vals[s] = variable v at point p0 + ps*s of slot ci.  Variables 0-7 are
+// read in pairs, one 16-byte (FP32: 8-byte) access per pair and point.
+template <typename S> struct PairOf;
+template <> struct PairOf<double> { using type = double2; };
+template <> struct PairOf<float> { using type = float2; };
+template <int N, typename R, typename S, typename F>
+__device__ __forceinline__ void line_vals(const S *q, int p0, int ps, int ci, F &&f) {
+  using S2 = typename PairOf<S>::type;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    R va[N], vb[N];
+#pragma unroll
+    for (int s = 0; s < N; ++s) {
+      const S2 d = *reinterpret_cast<const S2 *>(q + qo<N>(2 * k, p0 + ps * s, ci));
+      va[s] = R(d.x);
+      vb[s] = R(d.y);
+    }
+    f(2 * k, va);
+    f(2 * k + 1, vb);
+  }
+  R vc[N];
+#pragma unroll
+  for (int s = 0; s < N; ++s) vc[s] = R(q[qo<N>(8, p0 + ps * s, ci)]);
+  f(8, vc);
+}
+
+// Variables V0..V1-1 of point p of slot ci of the state tile q:
+// load_vars puts variable v in out[v - V0], store_vars writes in[v - V0]
+// to it; both pairs inside the range in one 16-byte access.
+template <int N, int V0, int V1>
+__device__ __forceinline__ void load_vars(const double *q, int p, int ci, double *out) {
+#pragma unroll
+  for (int v = V0; v < V1; ++v) {
+    if (v < 8 && (v & 1) == 0 && v + 1 < V1) {
+      const double2 d = *reinterpret_cast<const double2 *>(q + qo<N>(v, p, ci));
+      out[v - V0] = d.x;
+      out[v + 1 - V0] = d.y;
+    } else if (!(v < 8 && (v & 1) == 1 && v - 1 >= V0)) {
+      out[v - V0] = q[qo<N>(v, p, ci)];
+    }
+  }
+}
+template <int N, int V0, int V1>
+__device__ __forceinline__ void store_vars(double *q, int p, int ci, const double *in) {
+#pragma unroll
+  for (int v = V0; v < V1; ++v) {
+    if (v < 8 && (v & 1) == 0 && v + 1 < V1) {
+      *reinterpret_cast<double2 *>(q + qo<N>(v, p, ci)) = double2{in[v - V0], in[v + 1 - V0]};
+    } else if (!(v < 8 && (v & 1) == 1 && v - 1 >= V0)) {
+      q[qo<N>(v, p, ci)] = in[v - V0];
+    }
   }
 }
 
@@ -273,19 +330,15 @@ __device__ __forceinline__ void face_gather_w(const StageArgs &a, const OP &ops,
     if (ntile == tile) {
       lci = nbase - tile * QT;
     } else {
+      const int nci = nbase - ntile * QT;
       int p0, stride;
       bool endN;
       neighbour_line<N>(info & 3, nfp, p0, stride, endN);
       R Lr[N];
 #pragma unroll
       for (int s = 0; s < N; ++s) Lr[s] = endN ? ops.L[N][s] : ops.L[0][s];
-#pragma unroll
-      for (int v = 0; v < 9; ++v) {
-        R vals[N];
-#pragma unroll
-        for (int s = 0; s < N; ++s) vals[s] = R(a.qin[nbase + (v * N2 + p0 + stride * s) * T]);
-        Qn[v] = line_trace<N, R>(vals, Lr);
-      }
+      line_vals<N, R>(a.qin + (size_t)ntile * QT, p0, stride, nci,
+                      [&](int v, const R *vals) { Qn[v] = line_trace<N, R>(vals, Lr); });
     }
   }
 }
@@ -316,13 +369,9 @@ __device__ __forceinline__ void face_gather_pair(const StageArgs &a, const OP &o
     R Lr[N];
 #pragma unroll
     for (int s = 0; s < N; ++s) Lr[s] = endN ? ops.L[N][s] : ops.L[0][s];
-#pragma unroll
-    for (int v = 0; v < 9; ++v) {
-      R vals[N];
-#pragma unroll
-      for (int s = 0; s < N; ++s) vals[s] = R(a.qin[nbase + (v * N2 + p0 + stride * s) * T]);
-      Qn[v] = line_trace<N, R>(vals, Lr);
-    }
+    const int nt = nbase / QT;
+    line_vals<N, R>(a.qin + (size_t)nt * QT, p0, stride, nbase - nt * QT,
+                    [&](int v, const R *vals) { Qn[v] = line_trace<N, R>(vals, Lr); });
   };
   if (g0 || gN) {
     R Q[9];
@@ -365,13 +414,8 @@ __device__ __forceinline__ void face_trace_lds(const R *qs, const OP &ops, int i
   R Lr[N];
 #pragma unroll
   for (int s = 0; s < N; ++s) Lr[s] = endN ? ops.L[N][s] : ops.L[0][s];
-#pragma unroll
-  for (int v = 0; v < 9; ++v) {
-    R vals[N];
-#pragma unroll
-    for (int s = 0; s < N; ++s) vals[s] = qs[(v * N2 + p0 + stride * s) * T + lci];
-    Qn[v] = line_trace<N, R>(vals, Lr);
-  }
+  line_vals<N, R>(qs, p0, stride, lci,
+                  [&](int v, const R *vals) { Qn[v] = line_trace<N, R>(vals, Lr); });
 }
 
 // Flux at an end face point: Rusanov in the reference's left/right roles
@@ -476,7 +520,7 @@ __global__ __launch_bounds__(STAGE_THREADS, 2) void stage_kernel(StageArgs a, Op
   const int p0 = fam ? line : N * line;      // point of s = 0 on this line
   const int ps = fam ? N : 1;                // point stride along the line
   const int mx = fam ? G::S2X : G::S1X, my = fam ? G::S2Y : G::S1Y;
-  const double *ql = lds + LD::Q0 + p0 * T + ci;   // line values: ql[(v*N2 + ps*s)*T]
+  const double *qs = lds + LD::Q0;           // the staged tile (qo layout)
   // ---- neighbour traces at both end faces: gathered from memory for a
   //      neighbour in another tile, formed from the staged tile after the
   //      barrier for a neighbour in this one --------------------------------------
@@ -515,13 +559,9 @@ __global__ __launch_bounds__(STAGE_THREADS, 2) void stage_kernel(StageArgs a, Op
     double F0[9], wb0, wj0;
     {
       double t0[9];
-#pragma unroll
-      for (int v = 0; v < 9; ++v) {
-        double vals[N];
-#pragma unroll
-        for (int s = 0; s < N; ++s) vals[s] = ql[(v * N2 + ps * s) * T];
+      line_vals<N, double>(qs, p0, ps, ci, [&](int v, const double *vals) {
         t0[v] = line_trace<N, double>(vals, ops.L[0]);
-      }
+      });
       const double sx = msx[0], sy = msy[0];
       double Qv[9];
       face_flux<N, double>(info0, fam, 0, ph, c_h, t0, qn0, sx, sy, F0, Qv, eig, um);
@@ -537,13 +577,9 @@ __global__ __launch_bounds__(STAGE_THREADS, 2) void stage_kernel(StageArgs a, Op
     __builtin_amdgcn_sched_barrier(0);
     {
       double tN[9];
-#pragma unroll
-      for (int v = 0; v < 9; ++v) {
-        double vals[N];
-#pragma unroll
-        for (int s = 0; s < N; ++s) vals[s] = ql[(v * N2 + ps * s) * T];
+      line_vals<N, double>(qs, p0, ps, ci, [&](int v, const double *vals) {
         tN[v] = line_trace<N, double>(vals, ops.L[N]);
-      }
+      });
       const double sx = msx[N], sy = msy[N];
       double FN[9], Qv[9];
       face_flux<N, double>(infoN, fam, N, ph, c_h, tN, qnN, sx, sy, FN, Qv, eig, um);
@@ -584,14 +620,10 @@ __global__ __launch_bounds__(STAGE_THREADS, 2) void stage_kernel(StageArgs a, Op
       __builtin_amdgcn_sched_barrier(0);
       const bool two = r + 1 < N;
       double tr[9], tr2[9];
-#pragma unroll
-      for (int v = 0; v < 9; ++v) {
-        double vals[N];
-#pragma unroll
-        for (int s = 0; s < N; ++s) vals[s] = ql[(v * N2 + ps * s) * T];
+      line_vals<N, double>(qs, p0, ps, ci, [&](int v, const double *vals) {
         tr[v] = line_trace<N, double>(vals, ops.L[r]);
         if (two) tr2[v] = line_trace<N, double>(vals, ops.L[r + 1]);
-      }
+      });
       interior_point(r, tr);
       if (two) interior_point(r + 1, tr2);
     }
@@ -608,7 +640,7 @@ __global__ __launch_bounds__(STAGE_THREADS, 2) void stage_kernel(StageArgs a, Op
   constexpr int VS = 5;
   static_assert(VS <= LD::XV && 9 - VS <= LD::XV, "exchange area holds one round");
   double ax[N][VS];
-  const double *aux0 = a.aux[0] + (size_t)tile * G::QT + ci;
+  const double *aux0 = a.aux[0] + (size_t)tile * G::QT;
   // 1/J at the thread's points, with them
   double jis[N];
   if (valid) {
@@ -621,14 +653,10 @@ __global__ __launch_bounds__(STAGE_THREADS, 2) void stage_kernel(StageArgs a, Op
   if (RK && NAUX == 1 && valid) {
     if (fam == 0) {
 #pragma unroll
-      for (int s = 0; s < N; ++s)
-#pragma unroll
-        for (int v = 0; v < VS; ++v) ax[s][v] = aux0[(v * N2 + p0 + ps * s) * T];
+      for (int s = 0; s < N; ++s) load_vars<N, 0, VS>(aux0, p0 + ps * s, ci, ax[s]);
     } else {
 #pragma unroll
-      for (int s = 0; s < N; ++s)
-#pragma unroll
-        for (int v = VS; v < 9; ++v) ax[s][v - VS] = aux0[(v * N2 + p0 + ps * s) * T];
+      for (int s = 0; s < N; ++s) load_vars<N, VS, 9>(aux0, p0 + ps * s, ci, ax[s]);
     }
   }
 
@@ -684,7 +712,7 @@ __global__ __launch_bounds__(STAGE_THREADS, 2) void stage_kernel(StageArgs a, Op
     for (int v = 0; v < 9; ++v) {
       double vals[N], qv[NF];
 #pragma unroll
-      for (int s = 0; s < N; ++s) vals[s] = ql[(v * N2 + ps * s) * T];
+      for (int s = 0; s < N; ++s) vals[s] = qs[qo<N>(v, p0 + ps * s, ci)];
       qv[0] = qe[0][v];
       qv[N] = qe[1][v];
 #pragma unroll
@@ -717,19 +745,16 @@ __global__ __launch_bounds__(STAGE_THREADS, 2) void stage_kernel(StageArgs a, Op
       const int p = p0 + ps * s;
       const double ji = jis[s];
       double R[9], q[9];
-#pragma unroll
-      for (int v = 0; v < 9; ++v) {
-        // the velocity and the field enter the sources of every variable;
-        // the energy and psi only their own update
-        if (v < 4 || (v >= 5 && v < 8) || (v >= V0 && v < V1))
-          q[v] = lds[LD::Q0 + (v * N2 + p) * T + ci];
-      }
+      // the velocity and the field enter the sources of every variable;
+      // the energy and psi only their own update
+      load_vars<N, 0, 8>(qs, p, ci, q);
+      if (V1 == 9) q[8] = qs[qo<N>(8, p, ci)];
 #pragma unroll
       for (int v = V0; v < V1; ++v) {
         // resid = -(dF/dxi + dG/deta) / J   (compresid.f90:41,56), with the
         // viscous divergence added before the scaling (compresid.f90:43-46)
         if (VIS)
-          R[v] = (a.visres[(size_t)tile * G::QT + ci + (size_t)(v * N2 + p) * T] - D[s][v]) * ji;
+          R[v] = (a.visres[(size_t)tile * G::QT + qo<N>(v, p, ci)] - D[s][v]) * ji;
         else
           R[v] = -D[s][v] * ji;
       }
@@ -765,7 +790,7 @@ __global__ __launch_bounds__(STAGE_THREADS, 2) void stage_kernel(StageArgs a, Op
       }
       if (DIVB && V0 == 0) a.divb[(size_t)tile * N2 * T + (size_t)p * T + ci] = divB;
       if (RK) {
-        const size_t off = (size_t)tile * G::QT + ci;
+        const size_t off = (size_t)tile * G::QT;
         // several registers (FiveStageRk's last stage): their weighted sum,
         // formed before the point's stores (the output may be one of them)
         double xs[9];
@@ -774,22 +799,23 @@ __global__ __launch_bounds__(STAGE_THREADS, 2) void stage_kernel(StageArgs a, Op
           for (int v = V0; v < V1; ++v) xs[v] = 0.0;
 #pragma unroll
           for (int i = 0; i < NAUX; ++i) {
+            double xv[9];
+            load_vars<N, V0, V1>(a.aux[i] + off, p, ci, xv);
 #pragma unroll
-            for (int v = V0; v < V1; ++v)
-              xs[v] = fma(a.ca[i], a.aux[i][off + (size_t)(v * N2 + p) * T], xs[v]);
+            for (int v = V0; v < V1; ++v) xs[v] = fma(a.ca[i], xv[v - V0], xs[v]);
             __builtin_amdgcn_sched_barrier(0);   // one register's loads in flight at a time
           }
         }
+        double out[9];
 #pragma unroll
         for (int v = V0; v < V1; ++v) {
-          const size_t o = off + (size_t)(v * N2 + p) * T;
           double acc = a.cin * q[v];
           if (NAUX == 1) acc = fma(a.ca[0], ax[s][v - V0], acc);
           if (NAUX > 1) acc += xs[v];
-          acc = fma(cr, R[v], acc);
-          a.qout[o] = acc;
-          if (RSTORE) a.rstore[o] = R[v];
+          out[v - V0] = fma(cr, R[v], acc);
         }
+        store_vars<N, V0, V1>(a.qout + off, p, ci, out);
+        if (RSTORE) store_vars<N, V0, V1>(a.rstore + off, p, ci, R + V0);
       }
     }
   };
@@ -809,7 +835,7 @@ __global__ __launch_bounds__(STAGE_THREADS, 2) void stage_kernel(StageArgs a, Op
   //      reduction barrier so that it overlaps the xi-line wave's points:
   //      rho from the staged state, Jz^2 from the points step ------------------------
   if (AV && fam == 1)
-    av_sensor_lines<N, T, false>(a, lds + LD::Q0, lds + LD::XJ, lds + LD::A_ROW, rho_s, jz2_s,
+    av_sensor_lines<N, T, false, 2>(a, lds + LD::Q0 + qo<N>(0, 0, 0), lds + LD::XJ, lds + LD::A_ROW, rho_s, jz2_s,
                                  line, ci, cell, valid);
 
   // ---- block reductions: running maxima (getrusanov.f90:116-123,188-191) and

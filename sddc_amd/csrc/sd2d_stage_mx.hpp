@@ -118,7 +118,7 @@ void stage_kernel_mx(StageArgs a, OpsF ops, const float *__restrict__ metf) {
   const int p0 = fam ? line : N * line;
   const int ps = fam ? N : 1;
   const int mx = fam ? G::S2X : G::S1X, my = fam ? G::S2Y : G::S1Y;
-  const float *ql = ldf + LD::QF + p0 * T + ci;
+  const float *qsf = ldf + LD::QF;            // the staged tile (qo layout, FP32)
   // ---- neighbour traces at both end faces (memory or the staged tile) -------
   float qn0[9], qnN[9];
   int lci0 = -1, lciN = -1;
@@ -153,13 +153,9 @@ void stage_kernel_mx(StageArgs a, OpsF ops, const float *__restrict__ metf) {
     float F0[9], wb0, wj0;
     {
       float t0[9];
-#pragma unroll
-      for (int v = 0; v < 9; ++v) {
-        float vals[N];
-#pragma unroll
-        for (int s = 0; s < N; ++s) vals[s] = ql[(v * N2 + ps * s) * T];
+      line_vals<N, float>(qsf, p0, ps, ci, [&](int v, const float *vals) {
         t0[v] = line_trace<N, float>(vals, ops.L[0]);
-      }
+      });
       const float sx = msx[0], sy = msy[0];
       float Qv[9];
       face_flux<N, float>(info0, fam, 0, ph, c_h, t0, qn0, sx, sy, F0, Qv, eig, um);
@@ -169,13 +165,9 @@ void stage_kernel_mx(StageArgs a, OpsF ops, const float *__restrict__ metf) {
     __builtin_amdgcn_sched_barrier(0);
     {
       float tN[9];
-#pragma unroll
-      for (int v = 0; v < 9; ++v) {
-        float vals[N];
-#pragma unroll
-        for (int s = 0; s < N; ++s) vals[s] = ql[(v * N2 + ps * s) * T];
+      line_vals<N, float>(qsf, p0, ps, ci, [&](int v, const float *vals) {
         tN[v] = line_trace<N, float>(vals, ops.L[N]);
-      }
+      });
       const float sx = msx[N], sy = msy[N];
       float FN[9], Qv[9];
       face_flux<N, float>(infoN, fam, N, ph, c_h, tN, qnN, sx, sy, FN, Qv, eig, um);
@@ -210,14 +202,10 @@ void stage_kernel_mx(StageArgs a, OpsF ops, const float *__restrict__ metf) {
       __builtin_amdgcn_sched_barrier(0);
       const bool two = r + 1 < N;
       float tr[9], tr2[9];
-#pragma unroll
-      for (int v = 0; v < 9; ++v) {
-        float vals[N];
-#pragma unroll
-        for (int s = 0; s < N; ++s) vals[s] = ql[(v * N2 + ps * s) * T];
+      line_vals<N, float>(qsf, p0, ps, ci, [&](int v, const float *vals) {
         tr[v] = line_trace<N, float>(vals, ops.L[r]);
         if (two) tr2[v] = line_trace<N, float>(vals, ops.L[r + 1]);
-      }
+      });
       interior_point(r, tr);
       if (two) interior_point(r + 1, tr2);
     }
@@ -230,9 +218,8 @@ void stage_kernel_mx(StageArgs a, OpsF ops, const float *__restrict__ metf) {
   //      its first point are loaded now; every later point's one point ahead
   //      of their use, before the stores of the point in hand ----------------------
   constexpr int VS = LD::XV;
-  const double *qg = a.qin + (size_t)tile * G::QT + ci;
-  const double *aux0 = (NAUX == 1 ? a.aux[0] : a.qin) + (size_t)tile * G::QT + ci;
-  auto needed = [](int v, int v0, int v1) { return v >= v0 && v < v1; };
+  const double *qg = a.qin + (size_t)tile * G::QT;
+  const double *aux0 = (NAUX == 1 ? a.aux[0] : a.qin) + (size_t)tile * G::QT;
   double qn[9], xn[VS], jis[N];
   if (valid) {
 #pragma unroll
@@ -240,13 +227,13 @@ void stage_kernel_mx(StageArgs a, OpsF ops, const float *__restrict__ metf) {
       const int is = fam ? line : s, js = fam ? s : line;
       jis[s] = mt[(G::JI + js + N * is) * T];
     }
-    const int v0 = fam ? VS : 0, v1 = fam ? 9 : VS;
-#pragma unroll
-    for (int v = 0; v < 9; ++v)
-      if (needed(v, v0, v1)) qn[v] = qg[(v * N2 + p0) * T];
-#pragma unroll
-    for (int v = 0; v < VS; ++v)
-      xn[v] = (NAUX == 1 && v0 + v < v1) ? aux0[((v0 + v) * N2 + p0) * T] : 0.0;
+    if (fam == 0) {
+      load_vars<N, 0, VS>(qg, p0, ci, qn);
+      if (NAUX == 1) load_vars<N, 0, VS>(aux0, p0, ci, xn);
+    } else {
+      load_vars<N, VS, 9>(qg, p0, ci, qn + VS);
+      if (NAUX == 1) load_vars<N, VS, 9>(aux0, p0, ci, xn);
+    }
   }
   // ---- exchange through the derivative area (one slot per variable):
   //      round A, eta -> xi: variables 0..VS-1 and the divB (Jz) weights;
@@ -300,25 +287,22 @@ void stage_kernel_mx(StageArgs a, OpsF ops, const float *__restrict__ metf) {
       const double ji = jis[s];
       double R[9], q[9], xc[VS];
 #pragma unroll
-      for (int v = 0; v < 9; ++v)
-        if (needed(v, V0, V1)) q[v] = qn[v];
+      for (int v = V0; v < V1; ++v) q[v] = qn[v];
       // the velocity and the field of the sources, from the staged state
-      const float *st = ldf + LD::QF + p * T + ci;
-      const double w0 = st[0], w1 = st[N2 * T], w2 = st[2 * N2 * T], w3 = st[3 * N2 * T];
-      const double w5 = st[5 * N2 * T], w6 = st[6 * N2 * T], w7 = st[7 * N2 * T];
+      const float2 f01 = *reinterpret_cast<const float2 *>(qsf + qo<N>(0, p, ci));
+      const float2 f23 = *reinterpret_cast<const float2 *>(qsf + qo<N>(2, p, ci));
+      const float2 f45 = *reinterpret_cast<const float2 *>(qsf + qo<N>(4, p, ci));
+      const float2 f67 = *reinterpret_cast<const float2 *>(qsf + qo<N>(6, p, ci));
+      const double w0 = f01.x, w1 = f01.y, w2 = f23.x, w3 = f23.y;
+      const double w5 = f45.y, w6 = f67.x, w7 = f67.y;
 #pragma unroll
       for (int v = 0; v < V1 - V0; ++v) xc[v] = xn[v];
 #pragma unroll
       for (int v = V0; v < V1; ++v) R[v] = -double(D[s][v]) * ji;   // compresid.f90:41,56
       const double divB = double(db[s]) * ji;
       if (s + 1 < N) {
-#pragma unroll
-        for (int v = 0; v < 9; ++v)
-          if (needed(v, V0, V1)) qn[v] = qg[(v * N2 + p + ps) * T];
-        if (NAUX == 1) {
-#pragma unroll
-          for (int v = V0; v < V1; ++v) xn[v - V0] = aux0[(v * N2 + p + ps) * T];
-        }
+        load_vars<N, V0, V1>(qg, p + ps, ci, qn + V0);
+        if (NAUX == 1) load_vars<N, V0, V1>(aux0, p + ps, ci, xn);
       }
       const double ri = 1.0 / w0;
       const double ux = w1 * ri, uy = w2 * ri, uz = w3 * ri;
@@ -343,29 +327,30 @@ void stage_kernel_mx(StageArgs a, OpsF ops, const float *__restrict__ metf) {
         for (int v = V0; v < V1; ++v) nres = fma(R[v], R[v], nres);
         if (V0 == 0) ndiv = fma(divB, divB, ndiv);
       }
-      const size_t off = (size_t)tile * G::QT + ci;
+      const size_t off = (size_t)tile * G::QT;
       double xs[9];
       if (NAUX > 1) {
 #pragma unroll
         for (int v = V0; v < V1; ++v) xs[v] = 0.0;
 #pragma unroll
         for (int i = 0; i < NAUX; ++i) {
+          double xv[9];
+          load_vars<N, V0, V1>(a.aux[i] + off, p, ci, xv);
 #pragma unroll
-          for (int v = V0; v < V1; ++v)
-            xs[v] = fma(a.ca[i], a.aux[i][off + (size_t)(v * N2 + p) * T], xs[v]);
+          for (int v = V0; v < V1; ++v) xs[v] = fma(a.ca[i], xv[v - V0], xs[v]);
           __builtin_amdgcn_sched_barrier(0);   // one register's loads in flight at a time
         }
       }
+      double out[9];
 #pragma unroll
       for (int v = V0; v < V1; ++v) {
-        const size_t o = off + (size_t)(v * N2 + p) * T;
         double acc = a.cin * q[v];
         if (NAUX == 1) acc = fma(a.ca[0], xc[v - V0], acc);
         if (NAUX > 1) acc += xs[v];
-        acc = fma(cr, R[v], acc);
-        a.qout[o] = acc;
-        if (RSTORE) a.rstore[o] = R[v];
+        out[v - V0] = fma(cr, R[v], acc);
       }
+      store_vars<N, V0, V1>(a.qout + off, p, ci, out);
+      if (RSTORE) store_vars<N, V0, V1>(a.rstore + off, p, ci, R + V0);
     }
   };
   if (valid) {
@@ -416,7 +401,7 @@ void stage_kernel_mx(StageArgs a, OpsF ops, const float *__restrict__ metf) {
   // ---- AV smoothness sensor (AV.f90:14-83), in FP64, per cell, by the
   //      xi-line wave from the FP64 rho and Jz^2 of the points step -----------------
   if (AV && fam == 0)
-    av_sensor_lines<N, T, true>(a, lds + LD::A_RHO, lds + LD::A_JZ, lds + LD::A_ROW, rho_s, jz2_s, line,
+    av_sensor_lines<N, T, true, 1>(a, lds + LD::A_RHO, lds + LD::A_JZ, lds + LD::A_ROW, rho_s, jz2_s, line,
                          ci, tile * T + ci, valid);
 }
 

@@ -44,7 +44,7 @@ struct VisArgs {
   const double *metw;      // other-family metrics at the line ends, for walls
   double *gq;              // gradient [tile][(v*2+d)*N2 + p][ci]
   double *dei;             // d(e)/dxi, d(e)/deta [tile][k*N2 + p][ci] (walls)
-  double *visres;          // dFv + dGv [tile][v*N2 + p][ci]
+  double *visres;          // dFv + dGv, state tile layout (qo)
   const int *s2c;          // slot -> mesh cell, -1 for a padding slot
   double mu, eta0;
   int ncell, ntile, walls;
@@ -118,6 +118,21 @@ __device__ __forceinline__ void vis_trace(const double *f, int c0, int p0, int p
   }
 }
 
+// Trace of the 9 state variables at flux point row Lr of the line through
+// point p0 (stride ps) of slot ci of the state tile q (qo layout).
+template <int N>
+__device__ __forceinline__ void vis_trace_q(const double *q, int ci, int p0, int ps,
+                                            const double *Lr, double *out) {
+#pragma unroll
+  for (int v = 0; v < 9; ++v) {
+    double vals[N];
+#pragma unroll
+    for (int s = 0; s < N; ++s) vals[s] = q[qo<N>(v, p0 + ps * s, ci)];
+    out[v] = line_trace<N, double>(vals, Lr);
+    __builtin_amdgcn_sched_barrier(0);   // one variable's loads in flight at a time
+  }
+}
+
 // Averaged state at flux point r of a line (compflux.f90:71-76,89-94 in the
 // cell, the face/boundary average at the two ends), all read from memory.
 template <int N>
@@ -130,7 +145,7 @@ __device__ __forceinline__ void vis_point_state(const VisArgs &va, const Ops &op
 #pragma unroll
   for (int s = 0; s < N; ++s) Lr[s] = ops.L[r][s];
   double t[9];
-  vis_trace<N, 9>(va.qin + (size_t)tile * QT + ci, 0, p0, ps, Lr, t);
+  vis_trace_q<N>(va.qin + (size_t)tile * QT, ci, p0, ps, Lr, t);
   if (r != 0 && r != N) {
 #pragma unroll
     for (int v = 0; v < 9; ++v) Qv[v] = t[v];
@@ -149,7 +164,8 @@ __device__ __forceinline__ void vis_point_state(const VisArgs &va, const Ops &op
     double Ln[N];
 #pragma unroll
     for (int s = 0; s < N; ++s) Ln[s] = endN ? ops.L[N][s] : ops.L[0][s];
-    vis_trace<N, 9>(va.qin + nbase, 0, np0, nst, Ln, qn);
+    const int nt = nbase / Geo<N>::QT;
+    vis_trace_q<N>(va.qin + (size_t)nt * Geo<N>::QT, nbase - nt * Geo<N>::QT, np0, nst, Ln, qn);
   }
   const int mx = fam ? Geo<N>::S2X : Geo<N>::S1X, my = fam ? Geo<N>::S2Y : Geo<N>::S1Y;
   const double *mt = va.met + (size_t)tile * Geo<N>::NM * T + ci;
@@ -346,7 +362,7 @@ __device__ __forceinline__ void vis_end_grad(const VisArgs &va, const Ops &ops, 
 #pragma unroll
     for (int s = 0; s < N; ++s) Lr[s] = ops.L[rfp][s];
     double Qo[9], dface;
-    vis_trace<N, 9>(va.qin + (size_t)tile * QT + ci, 0, p0, ps, Lr, Qo);
+    vis_trace_q<N>(va.qin + (size_t)tile * QT, ci, p0, ps, Lr, Qo);
     // d(e)/deta along a xi-line, d(e)/dxi along an eta-line
     vis_trace<N, 1>(va.dei + (size_t)tile * 2 * N2 * T + ci, 1 - fam, p0, ps, Lr, &dface);
     const double *mw = va.metw + (size_t)tile * VisGeo<N>::NW * T + ci;
@@ -421,7 +437,7 @@ __global__ __launch_bounds__(STAGE_THREADS) void vis_flux_kernel(VisArgs va, Ops
   }
   __syncthreads();
   if (!valid || fam != 0) return;
-  double *o = va.visres + (size_t)tile * G::QT + ci;
+  double *o = va.visres + (size_t)tile * G::QT;
   const int js = line;
 #pragma unroll
   for (int is = 0; is < N; ++is)
@@ -432,7 +448,7 @@ __global__ __launch_bounds__(STAGE_THREADS) void vis_flux_kernel(VisArgs va, Ops
       for (int r = 0; r < NF; ++r) dF = dF + lds[LD::at(0, js, r, v, ci)] * ops.M[r][is];
 #pragma unroll
       for (int r = 0; r < NF; ++r) dG = dG + lds[LD::at(1, is, r, v, ci)] * ops.M[r][js];
-      o[(size_t)(v * N2 + is + N * js) * T] = dF + dG;
+      o[qo<N>(v, is + N * js, ci)] = dF + dG;
     }
 }
 

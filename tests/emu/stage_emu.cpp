@@ -368,7 +368,7 @@ int main(int argc, char **argv) {
     const int sl = c.tb.c2s[cell], t = sl / CPW, ci = sl % CPW;
     for (int pp = 0; pp < N2; ++pp)
       for (int v = 0; v < 9; ++v)
-        c.q[0][(size_t)t * QT + (size_t)(v * N2 + pp) * CPW + ci] = Q[v + 9 * (pp + (size_t)N2 * cell)];
+        c.q[0][(size_t)t * QT + qo_rt(N, v, pp, ci)] = Q[v + 9 * (pp + (size_t)N2 * cell)];
   }
   for (int t = 0; t < NSLOT; ++t) c.slots[2 * t] = c.slots[2 * t + 1] = 0ull;
   c.slots[0] = (unsigned long long)__double_as_longlong(pr[12]);
@@ -417,7 +417,7 @@ int main(int argc, char **argv) {
       const int sl = c.tb.c2s[cell], t = sl / CPW, ci = sl % CPW;
       for (int pp = 0; pp < N2; ++pp)
         for (int v = 0; v < 9; ++v)
-          Qo[v + 9 * (pp + (size_t)N2 * cell)] = c.q[c.cur][(size_t)t * QT + (size_t)(v * N2 + pp) * CPW + ci];
+          Qo[v + 9 * (pp + (size_t)N2 * cell)] = c.q[c.cur][(size_t)t * QT + qo_rt(N, v, pp, ci)];
     }
     put(Qo.data(), n);
     put(c.gmax.data(), 2);
