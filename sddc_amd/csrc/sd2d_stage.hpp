@@ -99,8 +99,8 @@ __device__ __forceinline__ int tile_of_block(int b, int nt, int rev) {
   return (x < r) ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
 }
 
-// LDS of one workgroup (in doubles): the tile's state [v][p][ci], the
-// derivatives of the fluxes of up to XV variables [v][p][ci] handed from one
+// LDS of one workgroup (in doubles): the tile's state (qo layout), the
+// derivatives of the fluxes of up to XV variables (xo layout) handed from one
 // family to the other (eta -> xi: variables 0-4, then xi -> eta: 5-8), of
 // the divB weight [p][ci] and (AV) of the Jz weight [p][ci], then
 // reduction slots.  The AV sensor of the eta-line wave reads rho from the
@@ -271,6 +271,46 @@ __device__ __forceinline__ void store_vars(double *q, int p, int ci, const doubl
       *reinterpret_cast<double2 *>(q + qo<N>(v, p, ci)) = double2{in[v - V0], in[v + 1 - V0]};
     } else if (!(v < 8 && (v & 1) == 1 && v - 1 >= V0)) {
       q[qo<N>(v, p, ci)] = in[v - V0];
+    }
+  }
+}
+
+// The derivative exchange area of the stage kernels holds up to five
+// variables u = 0..4 of every point of the tile, pairs (0,1), (2,3) as
+// [u/2][p][ci][u%2] and u = 4 as [p][ci] after them.  xput writes w[0..U-1]
+// to variables 0..U-1 of point p of slot ci, xadd adds them into w, both
+// with one 16-byte (FP32: 8-byte) LDS access per pair.
+template <int N>
+__device__ constexpr int xo(int u, int p, int ci) {
+  constexpr int T = cpw_of(N), N2 = N * N;
+  return u < 4 ? (((u >> 1) * N2 + p) * T + ci) * 2 + (u & 1) : 4 * N2 * T + p * T + ci;
+}
+template <int N, int U, typename S>
+__device__ __forceinline__ void xput(S *x, int p, int ci, const S *w) {
+  using S2 = typename PairOf<S>::type;
+#pragma unroll
+  for (int u = 0; u < U; u += 2) {
+    if (u + 1 < U && u < 4) {
+      S2 d;
+      d.x = w[u];
+      d.y = w[u + 1];
+      *reinterpret_cast<S2 *>(x + xo<N>(u, p, ci)) = d;
+    } else {
+      x[xo<N>(u, p, ci)] = w[u];
+    }
+  }
+}
+template <int N, int U, typename S>
+__device__ __forceinline__ void xadd(const S *x, int p, int ci, S *w) {
+  using S2 = typename PairOf<S>::type;
+#pragma unroll
+  for (int u = 0; u < U; u += 2) {
+    if (u + 1 < U && u < 4) {
+      const S2 d = *reinterpret_cast<const S2 *>(x + xo<N>(u, p, ci));
+      w[u] += d.x;
+      w[u + 1] += d.y;
+    } else {
+      w[u] += x[xo<N>(u, p, ci)];
     }
   }
 }
@@ -673,8 +713,7 @@ __global__ __launch_bounds__(STAGE_THREADS, 2) void stage_kernel(StageArgs a, Op
 #pragma unroll
     for (int s = 0; s < N; ++s) {
       const int p = p0 + ps * s;
-#pragma unroll
-      for (int v = 0; v < VS; ++v) lds[LD::XD + (v * N2 + p) * T + ci] = D[s][v];
+      xput<N, VS>(lds + LD::XD, p, ci, D[s]);
       lds[LD::XB + p * T + ci] = db[s];
     }
   }
@@ -683,11 +722,9 @@ __global__ __launch_bounds__(STAGE_THREADS, 2) void stage_kernel(StageArgs a, Op
 #pragma unroll
     for (int s = 0; s < N; ++s) {
       const int p = N * line + s;
-#pragma unroll
-      for (int v = 0; v < VS; ++v) D[s][v] += lds[LD::XD + (v * N2 + p) * T + ci];
+      xadd<N, VS>(lds + LD::XD, p, ci, D[s]);
       db[s] += lds[LD::XB + p * T + ci];
-#pragma unroll
-      for (int v = VS; v < 9; ++v) lds[LD::XD + ((v - VS) * N2 + p) * T + ci] = D[s][v];
+      xput<N, 9 - VS>(lds + LD::XD, p, ci, D[s] + VS);
       lds[LD::XB + p * T + ci] = db[s];
       if (AV) lds[LD::XJ + p * T + ci] = dj[s];
     }
@@ -697,8 +734,7 @@ __global__ __launch_bounds__(STAGE_THREADS, 2) void stage_kernel(StageArgs a, Op
 #pragma unroll
     for (int s = 0; s < N; ++s) {
       const int p = p0 + ps * s;
-#pragma unroll
-      for (int v = VS; v < 9; ++v) D[s][v] = lds[LD::XD + ((v - VS) * N2 + p) * T + ci] + D[s][v];
+      xadd<N, 9 - VS>(lds + LD::XD, p, ci, D[s] + VS);
       db[s] = lds[LD::XB + p * T + ci];
       if (AV) dj[s] += lds[LD::XJ + p * T + ci];
     }

@@ -35,8 +35,8 @@ struct OpsF {
 template <int N, int MODE> struct StageLdsMx {
   static constexpr int T = Geo<N>::CPW, N2 = N * N, NL = N - 1;
   static constexpr bool AV = (MODE & M_AV) != 0;
-  // in floats: the staged state [v][p][ci]; the flux derivatives of up to
-  // XV variables [v][p][ci] handed from one family to the other (eta -> xi:
+  // in floats: the staged state (qo layout); the flux derivatives of up to
+  // XV variables (xo layout) handed from one family to the other (eta -> xi:
   // variables 0-4, then xi -> eta: 5-8); the divB and Jz derivatives [p][ci]
   static constexpr int XV = 5;
   static constexpr int QF = 0;
@@ -243,8 +243,7 @@ void stage_kernel_mx(StageArgs a, OpsF ops, const float *__restrict__ metf) {
 #pragma unroll
     for (int s = 0; s < N; ++s) {
       const int p = p0 + ps * s;
-#pragma unroll
-      for (int v = 0; v < VS; ++v) ldf[LD::XD + (v * N2 + p) * T + ci] = D[s][v];
+      xput<N, VS>(ldf + LD::XD, p, ci, D[s]);
       ldf[LD::XB + p * T + ci] = db[s];
       if (AV) ldf[LD::XJ + p * T + ci] = dj[s];
     }
@@ -254,14 +253,12 @@ void stage_kernel_mx(StageArgs a, OpsF ops, const float *__restrict__ metf) {
 #pragma unroll
     for (int s = 0; s < N; ++s) {
       const int p = N * line + s;
-#pragma unroll
-      for (int v = 0; v < VS; ++v) D[s][v] += ldf[LD::XD + (v * N2 + p) * T + ci];
+      xadd<N, VS>(ldf + LD::XD, p, ci, D[s]);
       db[s] += ldf[LD::XB + p * T + ci];
       if (AV) dj[s] += ldf[LD::XJ + p * T + ci];
       // round B into the slots of this point just read (only this thread
       // reads or writes the slots of its points)
-#pragma unroll
-      for (int v = VS; v < 9; ++v) ldf[LD::XD + ((v - VS) * N2 + p) * T + ci] = D[s][v];
+      xput<N, 9 - VS>(ldf + LD::XD, p, ci, D[s] + VS);
       ldf[LD::XB + p * T + ci] = db[s];
     }
   }
@@ -270,8 +267,7 @@ void stage_kernel_mx(StageArgs a, OpsF ops, const float *__restrict__ metf) {
 #pragma unroll
     for (int s = 0; s < N; ++s) {
       const int p = p0 + ps * s;
-#pragma unroll
-      for (int v = VS; v < 9; ++v) D[s][v] = ldf[LD::XD + ((v - VS) * N2 + p) * T + ci] + D[s][v];
+      xadd<N, 9 - VS>(ldf + LD::XD, p, ci, D[s] + VS);
       db[s] = ldf[LD::XB + p * T + ci];
     }
   }
