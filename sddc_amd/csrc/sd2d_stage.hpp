@@ -14,8 +14,8 @@
 //     (Jz, for the AV sensor) likewise.
 // The two families exchange their derivatives through LDS in two rounds
 // and split the solution points' work by variable: each xi-line thread
-// (fam 0) finishes variables 0-4, each eta-line thread (fam 1) variables
-// 5-8 at the N points of its line: residual, Powell and GLM sources, the RK
+// (fam 0) finishes variables 0-3, each eta-line thread (fam 1) variables
+// 4-8 at the N points of its line: residual, Powell and GLM sources, the RK
 // combination, norm partials.  The running maxima and norms are reduced
 // per workgroup; the AV smoothness sensor runs per cell on the last stage.
 //
@@ -101,12 +101,11 @@ __device__ __forceinline__ int tile_of_block(int b, int nt, int rev) {
 
 // LDS of one workgroup (in doubles): the tile's state (qo layout), the
 // derivatives of the fluxes of up to XV variables (xo layout) handed from one
-// family to the other (eta -> xi: variables 0-4, then xi -> eta: 5-8), of
+// family to the other (eta -> xi: variables 0-3, then xi -> eta: 4-8), of
 // the divB weight [p][ci] and (AV) of the Jz weight [p][ci], then
-// reduction slots.  The AV sensor of the eta-line wave reads rho from the
-// state area, keeps Jz^2 in the Jz area and its line sums in the fifth
-// variable of the exchange area, which the other wave no longer touches
-// once the second exchange round has begun.
+// reduction slots.  The AV sensor's wave reads rho from the state area and
+// keeps Jz^2, then its line sums, in the Jz area, which only that wave
+// uses after the exchange.
 template <int N, int MODE> struct StageLds {
   static constexpr int T = Geo<N>::CPW, N2 = N * N;
   static constexpr bool AV = (MODE & M_AV) != 0;
@@ -117,8 +116,7 @@ template <int N, int MODE> struct StageLds {
   static constexpr int XJ = XB + N2 * T;
   static constexpr int RED = XJ + (AV ? N2 * T : 0);
   static constexpr int TOTAL = RED + 4 * (STAGE_THREADS / 64);
-  static constexpr int A_ROW = XD + 4 * N2 * T;   // AV line sums [2][line][ci]
-  static_assert(XV == 5 && 2 * N <= N2, "AV line sums fit the fifth exchange variable");
+  static_assert(2 * N <= N2, "the AV line sums fit the Jz area");
 };
 
 template <int N, int MODE>
@@ -173,6 +171,7 @@ __device__ __forceinline__ void av_sensor_lines(const StageArgs &a, const double
                                                 const double (&rs)[N], const double (&js2)[N],
                                                 int line, int ci, int cell, bool valid) {
   wave_lds_sync();   // the cell's Jz^2 (and rho) of the other lines are in LDS
+  double e1 = 0.0, e2 = 0.0;
   if (valid) {
     double y1[N], y2[N];
 #pragma unroll
@@ -188,7 +187,6 @@ __device__ __forceinline__ void av_sensor_lines(const StageArgs &a, const double
       y1[m] = u1;
       y2[m] = u2;
     }
-    double e1 = 0.0, e2 = 0.0;
 #pragma unroll
     for (int m = 0; m < N; ++m) {
       double h1 = 0.0, h2 = 0.0;
@@ -204,6 +202,10 @@ __device__ __forceinline__ void av_sensor_lines(const StageArgs &a, const double
       e1 = fma(d1, d1, e1);
       e2 += (d2 * d2) / (js2[m] * js2[m]);
     }
+  }
+  // (lineb may be the area jz2 was read from: every lane's reads first)
+  wave_lds_sync();
+  if (valid) {
     lineb[line * T + ci] = e1;
     lineb[(N + line) * T + ci] = e2;
   }
@@ -677,9 +679,11 @@ __global__ __launch_bounds__(STAGE_THREADS, 2) void stage_kernel(StageArgs a, Op
   //      variables a thread finishes is loaded now, so that its latency hides
   //      behind the exchange (the output may be the same array, so these
   //      loads cannot move past its stores) -------------------------------------
-  constexpr int VS = 5;
+  constexpr int VS = 4;
   static_assert(VS <= LD::XV && 9 - VS <= LD::XV, "exchange area holds one round");
-  double ax[N][VS];
+  // the line family whose wave runs the AV sensor (1: the eta-lines)
+  constexpr int AVF = 0;
+  double ax[N][VS > 9 - VS ? VS : 9 - VS];
   const double *aux0 = a.aux[0] + (size_t)tile * G::QT;
   // 1/J at the thread's points, with them
   double jis[N];
@@ -702,19 +706,21 @@ __global__ __launch_bounds__(STAGE_THREADS, 2) void stage_kernel(StageArgs a, Op
 
   // ---- exchange of the derivatives through LDS.  Round A: the eta-line
   //      threads hand variables 0..VS-1 and the divB weights to the xi-line
-  //      threads.  Round B: the xi-line threads hand variables VS..8, the
-  //      summed divB weight and (AV) their Jz weights to the eta-line
-  //      threads.  Each LDS location is read and then rewritten by the same
-  //      xi-line lane, so one area serves both rounds.  Afterwards D of each
-  //      thread holds the sum of both families for the variables it
-  //      finishes, db the summed divB weight, dj (eta-lines, AV) the summed
-  //      Jz weight -----------------------------------------------------------------
+  //      threads.  Round B: the xi-line threads hand variables VS..8 and the
+  //      summed divB weight to the eta-line threads.  The Jz weights (AV)
+  //      go to the family that runs the sensor, in round B to the eta-lines
+  //      (round A to the xi-lines).  Each LDS location is read and then
+  //      rewritten by the same xi-line lane, so one area serves both rounds.
+  //      Afterwards D of each thread holds the sum of both families for the
+  //      variables it finishes, db the summed divB weight, dj (the sensor's
+  //      family, AV) the summed Jz weight -----------------------------------------
   if (valid && fam == 1) {
 #pragma unroll
     for (int s = 0; s < N; ++s) {
       const int p = p0 + ps * s;
       xput<N, VS>(lds + LD::XD, p, ci, D[s]);
       lds[LD::XB + p * T + ci] = db[s];
+      if (AV && AVF == 0) lds[LD::XJ + p * T + ci] = dj[s];
     }
   }
   __syncthreads();
@@ -724,9 +730,10 @@ __global__ __launch_bounds__(STAGE_THREADS, 2) void stage_kernel(StageArgs a, Op
       const int p = N * line + s;
       xadd<N, VS>(lds + LD::XD, p, ci, D[s]);
       db[s] += lds[LD::XB + p * T + ci];
+      if (AV && AVF == 0) dj[s] += lds[LD::XJ + p * T + ci];
       xput<N, 9 - VS>(lds + LD::XD, p, ci, D[s] + VS);
       lds[LD::XB + p * T + ci] = db[s];
-      if (AV) lds[LD::XJ + p * T + ci] = dj[s];
+      if (AV && AVF == 1) lds[LD::XJ + p * T + ci] = dj[s];
     }
   }
   __syncthreads();
@@ -736,7 +743,7 @@ __global__ __launch_bounds__(STAGE_THREADS, 2) void stage_kernel(StageArgs a, Op
       const int p = p0 + ps * s;
       xadd<N, 9 - VS>(lds + LD::XD, p, ci, D[s] + VS);
       db[s] = lds[LD::XB + p * T + ci];
-      if (AV) dj[s] += lds[LD::XJ + p * T + ci];
+      if (AV && AVF == 1) dj[s] += lds[LD::XJ + p * T + ci];
     }
   }
 
@@ -806,7 +813,7 @@ __global__ __launch_bounds__(STAGE_THREADS, 2) void stage_kernel(StageArgs a, Op
       if (V0 <= 6 && 6 < V1) R[6] -= divB * uy;
       if (V0 <= 7 && 7 < V1) R[7] -= divB * uz;
       if (V0 <= 8 && 8 < V1) R[8] -= ph.alpha * q[8];
-      if (AV && V0 == VS) {
+      if (AV && V0 == (AVF == 0 ? 0 : VS)) {
         // the point's Jz^2 replaces its Jz weight in LDS (this thread read
         // that slot in the exchange; the sensor reads it back per cell)
         const double jz = dj[s] * ji;
@@ -866,13 +873,15 @@ __global__ __launch_bounds__(STAGE_THREADS, 2) void stage_kernel(StageArgs a, Op
     if (valid && fam == 1) compat_gradient(true);
   }
 
-  // ---- AV smoothness sensor (AV.f90:14-83), per cell, by the eta-line wave
-  //      (the one with fewer variables in the points step), before the
-  //      reduction barrier so that it overlaps the xi-line wave's points:
-  //      rho from the staged state, Jz^2 from the points step ------------------------
-  if (AV && fam == 1)
-    av_sensor_lines<N, T, false, 2>(a, lds + LD::Q0 + qo<N>(0, 0, 0), lds + LD::XJ, lds + LD::A_ROW, rho_s, jz2_s,
-                                 line, ci, cell, valid);
+  // ---- AV smoothness sensor (AV.f90:14-83), per cell, by the wave of the
+  //      family AVF (the xi-lines, which finish fewer variables in the points
+  //      step), before the reduction barrier so that it overlaps the other
+  //      wave's points: rho from the staged state, Jz^2 from the points step
+  //      in the Jz area, which then takes the line sums (only this wave uses
+  //      it after the exchange) --------------------------------------------------------
+  if (AV && fam == AVF)
+    av_sensor_lines<N, T, AVF == 0, 2>(a, lds + LD::Q0 + qo<N>(0, 0, 0), lds + LD::XJ,
+                                       lds + LD::XJ, rho_s, jz2_s, line, ci, cell, valid);
 
   // ---- block reductions: running maxima (getrusanov.f90:116-123,188-191) and
   //      norms (iterations.f90:37-46) ----------------------------------------------

@@ -37,7 +37,7 @@ template <int N, int MODE> struct StageLdsMx {
   static constexpr bool AV = (MODE & M_AV) != 0;
   // in floats: the staged state (qo layout); the flux derivatives of up to
   // XV variables (xo layout) handed from one family to the other (eta -> xi:
-  // variables 0-4, then xi -> eta: 5-8); the divB and Jz derivatives [p][ci]
+  // variables 0-3, then xi -> eta: 4-8); the divB and Jz derivatives [p][ci]
   static constexpr int XV = 5;
   static constexpr int QF = 0;
   static constexpr int XD = 9 * N2 * T;
@@ -217,10 +217,11 @@ void stage_kernel_mx(StageArgs a, OpsF ops, const float *__restrict__ metf) {
   //      The FP64 values of a thread's own variables and its RK register at
   //      its first point are loaded now; every later point's one point ahead
   //      of their use, before the stores of the point in hand ----------------------
-  constexpr int VS = LD::XV;
+  constexpr int VS = 4, VM = VS > 9 - VS ? VS : 9 - VS;
+  static_assert(VS <= LD::XV && 9 - VS <= LD::XV, "exchange area holds one round");
   const double *qg = a.qin + (size_t)tile * G::QT;
   const double *aux0 = (NAUX == 1 ? a.aux[0] : a.qin) + (size_t)tile * G::QT;
-  double qn[9], xn[VS], jis[N];
+  double qn[9], xn[VM], jis[N];
   if (valid) {
 #pragma unroll
     for (int s = 0; s < N; ++s) {   // 1/J at the thread's points
@@ -281,7 +282,7 @@ void stage_kernel_mx(StageArgs a, OpsF ops, const float *__restrict__ metf) {
     for (int s = 0; s < N; ++s) {
       const int p = p0 + ps * s;
       const double ji = jis[s];
-      double R[9], q[9], xc[VS];
+      double R[9], q[9], xc[VM];
 #pragma unroll
       for (int v = V0; v < V1; ++v) q[v] = qn[v];
       // the velocity and the field of the sources, from the staged state
