@@ -397,9 +397,9 @@ int sd2d_ens_nonfinite(sd2d_ens *ens, int32_t *flags);
  *               layouts; outputs may be NULL).
  * sd2d_adj_save / sd2d_adj_load  the checkpoints of the last sd2d_fwd as
  *               files, so that the backward pass can run in another
- *               context or process (SURVEY's restartable adjoint runs; no
- *               counterpart in the reference, whose restart files
- *               write_data.f90:44-74 they follow): <prefix>.NNNN.dat for
+ *               context or process, or after a crash (no counterpart in
+ *               the reference, whose restart files write_data.f90:44-74
+ *               they follow): <prefix>.NNNN.dat for
  *               checkpoint NNNN = 0..ceil(nsteps/ckpt), each a reference
  *               restart file with the maxima sidecar of
  *               sd2d_write_checkpoint, and <prefix>.adj (8 bytes "SD2DADJ1",
