@@ -42,6 +42,37 @@ def test_mixed_steps_against_oracle(name, k_stage):
                                [do["resid_rms"], do["divb_rms"]], rtol=1e-4)
 
 
+@pytest.mark.parametrize("name", ["ogrid4x8_N4", "channel7x6_N5", "periodic6x5_N4"])
+def test_mixed_av_sensor(name):
+    """The mixed kernel's AV sensor (FP64 rho and Jz^2 handed between the
+    lanes of one wave through LDS): its non-finite cells are the oracle's,
+    and it is bitwise the same in a second run and with another tile shape
+    (an LDS hand-over that raced would not repeat).  Its values follow the
+    mixed state, whose FP32 fluxes move small smoothness differences, so
+    they are compared with the oracle's only loosely (log10 within 4)."""
+    from sddc_amd.solver import Sd2d
+    from oracle.oracle import Oracle
+    print(f"[gpu-mixed] AV sensor {name}", flush=True)
+    case, Q0 = make_case(name)
+    res = []
+    for shape in (abi.SD2D_TILE_ROWPAIRS, abi.SD2D_TILE_ROWPAIRS, abi.SD2D_TILE_STRIPS):
+        g = Sd2d(case, precision=abi.SD2D_MIXED, tile_shape=shape)
+        g.set_state(Q0, [0, 0, 0, 0])
+        g.step(3, 2)
+        res.append(np.asarray(g.get_av()[0]))
+        g.close()
+    for other in res[1:]:
+        assert np.array_equal(res[0], other, equal_nan=True), name
+    o = Oracle(case)
+    o.set_state(Q0, [0, 0, 0, 0])
+    o.step(3, 2)
+    so = np.asarray(o.get_av()[0])
+    o.close()
+    fin = np.isfinite(so)
+    assert np.array_equal(np.isfinite(res[0]), fin), name
+    assert np.abs(res[0][fin] - so[fin]).max() <= 4.0, name
+
+
 def _l2_rel(a, b):
     """Per-variable relative L2 difference; w, Bz and psi (zero initially
     and O(rounding) throughout in 2-D) against the largest variable's norm."""
